@@ -1,0 +1,126 @@
+/*
+ * rv_sparse_hip.h -- MI355X extensions to the rv_sparse C API.
+ *
+ * Nothing in this header has a counterpart in rv-sparse: it adds what a GPU
+ * caller needs around the unchanged rv_sparse.h surface.
+ *   - device-resident operands for the descriptor and one-shot paths, so
+ *     callers that keep matrices in HBM skip the PCIe copies the host API
+ *     implies;
+ *   - the HIP stream a descriptor launches on, and hipGraph replay of the
+ *     numeric pass for descriptor reuse;
+ *   - a batched entry point that runs many independent products in one
+ *     persistent launch;
+ *   - small device-memory and timing helpers, so callers without a HIP
+ *     toolchain (C through dlopen, Python through ctypes) can stage data.
+ * All functions are extern "C", take plain pointers and sizes, and return
+ * rvsp_status_t. A missing or unusable GPU is reported as
+ * RVSP_ERROR_UNSUPPORTED_BACKEND; there is no CPU fallback.
+ */
+#ifndef RV_SPARSE_HIP_H
+#define RV_SPARSE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rv_sparse.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- device and memory helpers ---------------------------------------- */
+
+/* RVSP_SUCCESS when a HIP device is usable, RVSP_ERROR_UNSUPPORTED_BACKEND
+ * otherwise. *count (optional) receives the number of visible devices. */
+rvsp_status_t rvsp_hip_available(int *count);
+
+rvsp_status_t rvsp_hip_malloc(void **ptr, size_t bytes);
+rvsp_status_t rvsp_hip_free(void *ptr);
+rvsp_status_t rvsp_hip_memcpy_h2d(void *dst, const void *src, size_t bytes);
+rvsp_status_t rvsp_hip_memcpy_d2h(void *dst, const void *src, size_t bytes);
+rvsp_status_t rvsp_hip_memcpy_d2d(void *dst, const void *src, size_t bytes);
+rvsp_status_t rvsp_hip_memset(void *dst, int value, size_t bytes);
+/* Waits for all work on the device. */
+rvsp_status_t rvsp_hip_synchronize(void);
+
+/* Events on a given stream (NULL = the descriptor-less default stream). */
+typedef struct rvsp_hip_event *rvsp_hip_event_t;
+rvsp_status_t rvsp_hip_event_create(rvsp_hip_event_t *ev);
+rvsp_status_t rvsp_hip_event_record(rvsp_hip_event_t ev, void *stream);
+rvsp_status_t rvsp_hip_event_synchronize(rvsp_hip_event_t ev);
+rvsp_status_t rvsp_hip_event_elapsed_ms(rvsp_hip_event_t start,
+                                        rvsp_hip_event_t stop,
+                                        float *ms);
+void rvsp_hip_event_destroy(rvsp_hip_event_t ev);
+
+/* Phase timing (process-wide). When on, each analysis, structure fill and
+ * numeric pass records a pair of HIP events on the stream it runs on.
+ * rvsp_hip_get_timing waits for them and returns, per phase
+ * (0 analysis, 1 fill, 2 numeric, 3 unused), the summed milliseconds and
+ * the number of calls. */
+void rvsp_hip_set_timing(int on);
+rvsp_status_t rvsp_hip_get_timing(double *ms, long long *calls, int n);
+void rvsp_hip_reset_timing(void);
+
+/* ---- descriptor extensions --------------------------------------------- */
+
+typedef enum
+{
+    RVSP_MEMORY_HOST = 0,  /* operands and C hold host pointers (default) */
+    RVSP_MEMORY_DEVICE = 1 /* operands and C hold device pointers         */
+} rvsp_memory_t;
+
+/* Where A, B and C live for this descriptor. Call before
+ * work_estimation(); INVALID_ARGUMENT afterwards. With RVSP_MEMORY_DEVICE the
+ * pointers in A, B (and C at compute) must be device allocations that stay
+ * valid until the descriptor is destroyed or re-estimated, and compute()
+ * returns once the work is queued on the descriptor's stream (no host
+ * synchronisation); C->row_ptr then receives a device-to-device copy. */
+rvsp_status_t rvsp_spgemm_set_memory(rvsp_spgemm_descr_t descr,
+                                     rvsp_memory_t memory);
+
+/* Use an existing HIP stream (hipStream_t passed as void *). The library
+ * never destroys a stream it did not create. Call before work_estimation. */
+rvsp_status_t rvsp_spgemm_set_stream(rvsp_spgemm_descr_t descr, void *stream);
+
+/* The stream the descriptor launches on (created at descr_create unless set). */
+rvsp_status_t rvsp_spgemm_get_stream(rvsp_spgemm_descr_t descr, void **stream);
+
+/* Capture the numeric pass in a hipGraph on the first compute() that does
+ * not refill C's structure, and replay it on later calls whose pointers
+ * match. Default off. Only meaningful with RVSP_MEMORY_DEVICE. */
+rvsp_status_t rvsp_spgemm_set_graph(rvsp_spgemm_descr_t descr, int enable);
+
+/* Waits for the descriptor's queued work and reports a kernel-side
+ * consistency error (RVSP_ERROR_INVALID_CSR: the operands were not the
+ * canonical CSR the analysis saw). Needed after device-memory compute(). */
+rvsp_status_t rvsp_spgemm_wait(rvsp_spgemm_descr_t descr);
+
+/* Total intermediate products (sum of op counts) of the last estimation. */
+rvsp_status_t rvsp_spgemm_get_products(rvsp_spgemm_descr_t descr,
+                                       int64_t *products);
+
+/* ---- device one-shot ---------------------------------------------------- */
+
+/*
+ * C = A * B with A, B device-resident. C receives device arrays allocated by
+ * the library (owns_data = RVSP_OWNS_DEVICE); release them with
+ * rvsp_csr_destroy. No canonical check is made (the data is on the device);
+ * inputs must be canonical CSR. options as for rvsp_spgemm_csr, plus
+ * algo selection through rvsp_spgemm_csr_device_algo. Synchronous.
+ */
+rvsp_status_t rvsp_spgemm_csr_device(const rvsp_csr_matrix_t *A,
+                                     const rvsp_csr_matrix_t *B,
+                                     rvsp_csr_matrix_t *C,
+                                     const rvsp_spgemm_options_t *options);
+
+rvsp_status_t rvsp_spgemm_csr_device_algo(const rvsp_csr_matrix_t *A,
+                                          const rvsp_csr_matrix_t *B,
+                                          rvsp_csr_matrix_t *C,
+                                          rvsp_spgemm_algo_t algo);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RV_SPARSE_HIP_H */

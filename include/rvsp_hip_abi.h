@@ -1,0 +1,97 @@
+/*
+ * rvsp_hip_abi.h -- thin C ABI between the C host core (descriptor state,
+ * dispatch, host-pointer API) and the HIP kernels for gfx950.
+ *
+ * Internal to the library: no counterpart in rv-sparse. It takes device
+ * pointers and plain sizes only; HIP types cross it as void *. A "plan" is
+ * the device half of one SpGEMM structure: per-row product counts, the row
+ * bins the kernels are launched over, C's row pointer, and the scratch the
+ * numeric pass needs. It borrows the operand structure pointers given to
+ * rvsp_hip_plan_analyse until the next analyse or destroy.
+ */
+#ifndef RVSP_HIP_ABI_H
+#define RVSP_HIP_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rv_sparse_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rvsp_hip_plan rvsp_hip_plan_t;
+
+/* Row-bin selector used by the analysis. */
+typedef enum
+{
+    RVSP_SELECT_HEURISTIC = 0,
+    RVSP_SELECT_LEARNED = 1
+} rvsp_select_t;
+
+/* stream: an existing hipStream_t, or NULL to create (and own) one. */
+rvsp_status_t rvsp_hip_plan_create(rvsp_hip_plan_t **plan, void *stream);
+void rvsp_hip_plan_destroy(rvsp_hip_plan_t *plan);
+void *rvsp_hip_plan_stream(const rvsp_hip_plan_t *plan);
+
+/*
+ * Symbolic analysis of C = A * B (A m x k, B k x n) on device-resident
+ * structure. Fills per-row product counts, per-row nnz(C), C's row pointer
+ * (device) and the numeric bins. Blocks until nnz(C) is known.
+ * Returns RVSP_ERROR_ALLOCATION_FAILED when nnz(C) > INT32_MAX.
+ */
+rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *plan, int32_t m,
+                                    int32_t k, int32_t n, int32_t nnz_a,
+                                    const int32_t *a_row_ptr,
+                                    const int32_t *a_col_idx, int32_t nnz_b,
+                                    const int32_t *b_row_ptr,
+                                    const int32_t *b_col_idx,
+                                    rvsp_select_t select, int64_t *c_nnz,
+                                    int64_t *products);
+
+/* Device pointer to C's row pointer (m+1 entries) of the last analysis. */
+const int32_t *rvsp_hip_plan_row_ptr(const rvsp_hip_plan_t *plan);
+/* Device pointer to the per-row product counts (m entries). */
+const int64_t *rvsp_hip_plan_op_counts(const rvsp_hip_plan_t *plan);
+
+/* Writes C's ascending column indices (device). Queued on the plan stream. */
+rvsp_status_t rvsp_hip_plan_fill(rvsp_hip_plan_t *plan, int32_t *c_col_idx);
+
+/* Numeric pass: C's values (device) from the stored structure in
+ * c_col_idx. Queued on the plan stream. */
+rvsp_status_t rvsp_hip_plan_numeric(rvsp_hip_plan_t *plan,
+                                    const float *a_values,
+                                    const float *b_values,
+                                    const int32_t *c_col_idx,
+                                    float *c_values);
+
+/* Same numeric pass, captured once into a hipGraph and replayed while the
+ * four pointers stay the same (re-captured when they change). */
+rvsp_status_t rvsp_hip_plan_numeric_graph(rvsp_hip_plan_t *plan,
+                                          const float *a_values,
+                                          const float *b_values,
+                                          const int32_t *c_col_idx,
+                                          float *c_values);
+
+/* Checks the device-side overflow/consistency flag the kernels raise
+ * (table overflow, probe exhaustion). RVSP_ERROR_INVALID_CSR if raised;
+ * synchronises the plan stream. */
+rvsp_status_t rvsp_hip_plan_check(rvsp_hip_plan_t *plan);
+
+/* Stream helpers used by the host core. */
+rvsp_status_t rvsp_hip_stream_sync(void *stream);
+rvsp_status_t rvsp_hip_copy_async(void *dst, const void *src, size_t bytes,
+                                  void *stream);
+rvsp_status_t rvsp_hip_alloc_async(void **ptr, size_t bytes, void *stream);
+rvsp_status_t rvsp_hip_free_async(void *ptr, void *stream);
+
+/* Test hook: makes the next n device allocations fail, so cleanup paths
+ * can be exercised without exhausting memory. */
+void rvsp_hip_fail_next_allocs(int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RVSP_HIP_ABI_H */

@@ -1,0 +1,32 @@
+#!/usr/bin/env bash
+# Builds rv-sparse's own scalar library from its sources, where they lie, into
+# oracle/_ref/librvsp_ref.so. It is a second CPU checker next to oracle.c and
+# may serve as the timed CPU baseline. Only the reference's C sources are
+# compiled (no reference build system, no generated code, no stand-ins); the
+# three RISC-V Vector accumulators are left out exactly as rv-sparse's
+# Makefile does for a non-vector -march. -march=x86-64-v3 instead of the
+# reference's -march=native keeps the library runnable on any AVX2 host.
+# Skips quietly (exit 0) when the reference tree is absent.
+set -euo pipefail
+REF=${RVSP_REFERENCE_DIR:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+if [ ! -d "$REF/src" ]; then
+    echo "build_ref: $REF not present, skipping"
+    exit 0
+fi
+mkdir -p "$OUT/obj"
+CFLAGS="-std=c11 -O3 -march=x86-64-v3 -fPIC -I$REF/include -w"
+objs=()
+while IFS= read -r src; do
+    case "$src" in
+        */accum_rvv_f32.c|*/accum_contig_f32.c|*/accum_adaptive_f32.c) continue ;;
+    esac
+    obj="$OUT/obj/$(echo "${src#$REF/}" | tr '/' '_' ).o"
+    gcc $CFLAGS -c "$src" -o "$obj"
+    objs+=("$obj")
+done < <(find "$REF/src" -name '*.c' | sort)
+gcc -shared -Wl,-Bsymbolic -o "$OUT/librvsp_ref.so.tmp" "${objs[@]}" -lm
+mv "$OUT/librvsp_ref.so.tmp" "$OUT/librvsp_ref.so"
+rm -rf "$OUT/obj"
+echo "build_ref: wrote $OUT/librvsp_ref.so"

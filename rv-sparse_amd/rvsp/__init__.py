@@ -1,0 +1,520 @@
+"""ctypes binding of the MI355X rv_sparse C library (tests, bench, scripts).
+
+The product is the C-ABI shared library lib/librvsparse_hip.so (rv_sparse.h,
+rv_sparse_hip.h, rvsp_spgemm.h). This module only marshals numpy arrays in
+and out of it; every product runs inside that library on the GPU. Nothing
+here computes a SpGEMM, and a missing library or GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(os.path.dirname(_HERE), "lib")
+LIB_PATH = os.path.join(LIB_DIR, "librvsparse_hip.so")
+GEN_PATH = os.path.join(LIB_DIR, "librvsp_gen.so")
+
+# rv_sparse_types.h
+SUCCESS = 0
+ERROR_NULL_POINTER = -1
+ERROR_INVALID_ARGUMENT = -2
+ERROR_INVALID_CSR = -3
+ERROR_UNSUPPORTED_DTYPE = -4
+ERROR_UNSUPPORTED_BACKEND = -5
+ERROR_ALLOCATION_FAILED = -6
+
+DTYPE_INT8, DTYPE_INT32, DTYPE_BF16, DTYPE_FP32, DTYPE_FP64 = range(5)
+FORMAT_CSR = 0
+BACKEND_SCALAR, BACKEND_SCALAR_UNROLL4, BACKEND_RVV_INTRINSICS, BACKEND_HIP = range(4)
+ALGO_DEFAULT, ALGO_RVV, ALGO_CONTIG, ALGO_ADAPTIVE, ALGO_HIP, ALGO_HIP_LEARNED = range(6)
+MEMORY_HOST, MEMORY_DEVICE = 0, 1
+OWNS_DEVICE = 2
+
+CSR_OK, CSR_BAD_ROW_PTR, CSR_COL_OUT_OF_RANGE, CSR_UNSORTED, CSR_DUPLICATE = range(5)
+
+
+class RvspError(RuntimeError):
+    def __init__(self, where: str, status: int):
+        super().__init__(f"{where}: status {status} ({status_string(status)})")
+        self.status = status
+
+
+class CsrStruct(C.Structure):
+    """rvsp_csr_matrix_t, field for field."""
+
+    _fields_ = [
+        ("rows", C.c_int32),
+        ("cols", C.c_int32),
+        ("nnz", C.c_int32),
+        ("row_ptr", C.c_void_p),
+        ("col_idx", C.c_void_p),
+        ("values", C.c_void_p),
+        ("dtype", C.c_int),
+        ("format", C.c_int),
+        ("owns_data", C.c_int),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [("backend", C.c_int), ("input_dtype", C.c_int), ("output_dtype", C.c_int)]
+
+
+_lib = None
+_gen = None
+
+
+def _declare(lib):
+    P, I32, I64, SZ, V = C.POINTER, C.c_int32, C.c_int64, C.c_size_t, C.c_void_p
+    CSRP = P(CsrStruct)
+    sig = {
+        "rvsp_get_version": (C.c_char_p, []),
+        "rvsp_status_to_string": (C.c_char_p, [C.c_int]),
+        "rvsp_csr_create": (C.c_int, [CSRP, I32, I32, I32, V, V, V, C.c_int]),
+        "rvsp_csr_validate": (C.c_int, [CSRP]),
+        "rvsp_csr_destroy": (None, [CSRP]),
+        "rvsp_spgemm_csr": (C.c_int, [CSRP, CSRP, CSRP, P(Options)]),
+        "rvsp_spgemm_descr_create": (C.c_int, [P(V)]),
+        "rvsp_spgemm_descr_destroy": (None, [V]),
+        "rvsp_spgemm_set_algo": (C.c_int, [V, C.c_int]),
+        "rvsp_spgemm_work_estimation": (C.c_int, [V, CSRP, CSRP, P(SZ), P(I32)]),
+        "rvsp_spgemm_compute": (C.c_int, [V, CSRP, CSRP, CSRP, V]),
+        "rvsp_spgemm_get_op_counts": (C.c_int, [V, V, I32]),
+        "rvsp_spgemm_invalidate_structure": (None, [V]),
+        "rvsp_spgemm_buffer_size": (C.c_int, [I32, P(SZ)]),
+        "rvsp_csr_check": (C.c_int, [I32, I32, V, V, P(I32)]),
+        "rvsp_csr_status_string": (C.c_char_p, [C.c_int]),
+        "rvsp_spgemm_hip_f32": (
+            C.c_int,
+            [I32, I32, I32, V, V, V, V, V, V, V, SZ, P(V), P(V), P(V), P(I32), V],
+        ),
+        "rvsp_hip_available": (C.c_int, [P(C.c_int)]),
+        "rvsp_hip_malloc": (C.c_int, [P(V), SZ]),
+        "rvsp_hip_free": (C.c_int, [V]),
+        "rvsp_hip_memcpy_h2d": (C.c_int, [V, V, SZ]),
+        "rvsp_hip_memcpy_d2h": (C.c_int, [V, V, SZ]),
+        "rvsp_hip_memcpy_d2d": (C.c_int, [V, V, SZ]),
+        "rvsp_hip_memset": (C.c_int, [V, C.c_int, SZ]),
+        "rvsp_hip_synchronize": (C.c_int, []),
+        "rvsp_hip_event_create": (C.c_int, [P(V)]),
+        "rvsp_hip_event_record": (C.c_int, [V, V]),
+        "rvsp_hip_event_synchronize": (C.c_int, [V]),
+        "rvsp_hip_event_elapsed_ms": (C.c_int, [V, V, P(C.c_float)]),
+        "rvsp_hip_event_destroy": (None, [V]),
+        "rvsp_spgemm_set_memory": (C.c_int, [V, C.c_int]),
+        "rvsp_spgemm_set_stream": (C.c_int, [V, V]),
+        "rvsp_spgemm_get_stream": (C.c_int, [V, P(V)]),
+        "rvsp_spgemm_set_graph": (C.c_int, [V, C.c_int]),
+        "rvsp_spgemm_wait": (C.c_int, [V]),
+        "rvsp_spgemm_get_products": (C.c_int, [V, P(I64)]),
+        "rvsp_spgemm_csr_device": (C.c_int, [CSRP, CSRP, CSRP, P(Options)]),
+        "rvsp_spgemm_csr_device_algo": (C.c_int, [CSRP, CSRP, CSRP, C.c_int]),
+        "rvsp_hip_fail_next_allocs": (None, [C.c_int]),
+        "rvsp_hip_set_timing": (None, [C.c_int]),
+        "rvsp_hip_get_timing": (C.c_int, [P(C.c_double), P(C.c_longlong), C.c_int]),
+        "rvsp_hip_reset_timing": (None, []),
+        "rvsp_hip_alloc_async": (C.c_int, [P(V), SZ, V]),
+        "rvsp_hip_free_async": (C.c_int, [V, V]),
+        "rvsp_hip_stream_sync": (C.c_int, [V]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def lib():
+    """The loaded C library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build with __graft_entry__.build()")
+        _lib = _declare(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def status_string(status: int) -> str:
+    return lib().rvsp_status_to_string(status).decode()
+
+
+def check(where: str, status: int):
+    if status != SUCCESS:
+        raise RvspError(where, status)
+
+
+def gpu_available() -> bool:
+    n = C.c_int(0)
+    return lib().rvsp_hip_available(C.byref(n)) == SUCCESS
+
+
+# ---------------------------------------------------------------------------
+# host CSR
+# ---------------------------------------------------------------------------
+@dataclass
+class Csr:
+    """Host CSR: int32 row_ptr/col_idx, float32 values."""
+
+    rows: int
+    cols: int
+    row_ptr: np.ndarray
+    col_idx: np.ndarray
+    values: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_ptr[-1]) if len(self.row_ptr) else 0
+
+    def struct(self) -> CsrStruct:
+        for a, dt in ((self.row_ptr, np.int32), (self.col_idx, np.int32), (self.values, np.float32)):
+            assert a.dtype == dt and a.flags.c_contiguous
+        return CsrStruct(
+            self.rows,
+            self.cols,
+            self.nnz,
+            self.row_ptr.ctypes.data,
+            self.col_idx.ctypes.data,
+            self.values.ctypes.data,
+            DTYPE_FP32,
+            FORMAT_CSR,
+            0,
+        )
+
+    @staticmethod
+    def make(rows, cols, row_ptr, col_idx, values) -> "Csr":
+        return Csr(
+            int(rows),
+            int(cols),
+            np.ascontiguousarray(row_ptr, dtype=np.int32),
+            np.ascontiguousarray(col_idx, dtype=np.int32),
+            np.ascontiguousarray(values, dtype=np.float32),
+        )
+
+
+def _from_owned_host(c: CsrStruct) -> Csr:
+    """Copies a library-allocated host C into numpy and frees it."""
+    n = c.nnz
+    rp = np.ctypeslib.as_array(C.cast(c.row_ptr, C.POINTER(C.c_int32)), (c.rows + 1,)).copy()
+    if n > 0:
+        ci = np.ctypeslib.as_array(C.cast(c.col_idx, C.POINTER(C.c_int32)), (n,)).copy()
+        va = np.ctypeslib.as_array(C.cast(c.values, C.POINTER(C.c_float)), (n,)).copy()
+    else:
+        ci = np.zeros(0, np.int32)
+        va = np.zeros(0, np.float32)
+    out = Csr(c.rows, c.cols, rp, ci, va)
+    lib().rvsp_csr_destroy(C.byref(c))
+    return out
+
+
+def spgemm_csr(A: Csr, B: Csr, backend: int = BACKEND_HIP, raise_on_error=True):
+    """rvsp_spgemm_csr (one-shot, host memory). Returns (status, Csr|None)."""
+    a, b = A.struct(), B.struct()
+    c = CsrStruct()
+    opt = Options(backend, DTYPE_FP32, DTYPE_FP32)
+    st = lib().rvsp_spgemm_csr(C.byref(a), C.byref(b), C.byref(c), C.byref(opt))
+    if st != SUCCESS:
+        if raise_on_error:
+            raise RvspError("rvsp_spgemm_csr", st)
+        return st, None
+    return st, _from_owned_host(c)
+
+
+def spgemm_raw(A: Csr, B: Csr, want_ops=False):
+    """rvsp_spgemm_hip_f32, the raw one-shot kernel with callee-malloc'd C."""
+    L = lib()
+    ws_bytes = C.c_size_t(0)
+    check("buffer_size", L.rvsp_spgemm_buffer_size(B.cols, C.byref(ws_bytes)))
+    ws = np.zeros(ws_bytes.value, np.uint8)
+    rp, ci, va = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    nnz = C.c_int32(0)
+    ops = np.zeros(A.rows, np.int64) if want_ops else None
+    st = L.rvsp_spgemm_hip_f32(
+        A.rows, A.cols, B.cols,
+        A.row_ptr.ctypes.data, A.col_idx.ctypes.data, A.values.ctypes.data,
+        B.row_ptr.ctypes.data, B.col_idx.ctypes.data, B.values.ctypes.data,
+        ws.ctypes.data, ws_bytes.value,
+        C.byref(rp), C.byref(ci), C.byref(va), C.byref(nnz),
+        ops.ctypes.data if want_ops else None,
+    )
+    check("rvsp_spgemm_hip_f32", st)
+    c = CsrStruct(A.rows, B.cols, nnz.value, rp.value, ci.value, va.value, DTYPE_FP32, FORMAT_CSR, 1)
+    out = _from_owned_host(c)
+    return (out, ops) if want_ops else out
+
+
+class Descr:
+    """rvsp_spgemm_descr_t wrapper (host or device memory mode)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check("descr_create", lib().rvsp_spgemm_descr_create(C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().rvsp_spgemm_descr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_algo(self, algo) -> int:
+        return lib().rvsp_spgemm_set_algo(self.h, algo)
+
+    def set_memory(self, mem) -> int:
+        return lib().rvsp_spgemm_set_memory(self.h, mem)
+
+    def set_graph(self, on) -> int:
+        return lib().rvsp_spgemm_set_graph(self.h, int(on))
+
+    def stream(self):
+        s = C.c_void_p()
+        check("get_stream", lib().rvsp_spgemm_get_stream(self.h, C.byref(s)))
+        return s.value
+
+    def estimate(self, a: CsrStruct, b: CsrStruct):
+        ws = C.c_size_t(0)
+        nnz = C.c_int32(0)
+        st = lib().rvsp_spgemm_work_estimation(self.h, C.byref(a), C.byref(b), C.byref(ws), C.byref(nnz))
+        return st, ws.value, nnz.value
+
+    def compute(self, a: CsrStruct, b: CsrStruct, c: CsrStruct, ws) -> int:
+        return lib().rvsp_spgemm_compute(self.h, C.byref(a), C.byref(b), C.byref(c), ws)
+
+    def op_counts(self, n):
+        out = np.zeros(max(n, 1), np.int64)
+        st = lib().rvsp_spgemm_get_op_counts(self.h, out.ctypes.data, n)
+        return st, out[:n]
+
+    def products(self) -> int:
+        v = C.c_int64(0)
+        check("get_products", lib().rvsp_spgemm_get_products(self.h, C.byref(v)))
+        return v.value
+
+    def wait(self) -> int:
+        return lib().rvsp_spgemm_wait(self.h)
+
+    def invalidate(self):
+        lib().rvsp_spgemm_invalidate_structure(self.h)
+
+
+# ---------------------------------------------------------------------------
+# device memory
+# ---------------------------------------------------------------------------
+class DeviceBuffer:
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        check("hip_malloc", lib().rvsp_hip_malloc(C.byref(p), max(int(nbytes), 1)))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    @staticmethod
+    def from_array(a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        d = DeviceBuffer(a.nbytes)
+        check("h2d", lib().rvsp_hip_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check("h2d", lib().rvsp_hip_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes))
+
+    def to_array(self, dtype, count) -> np.ndarray:
+        out = np.empty(count, dtype)
+        if count:
+            check("d2h", lib().rvsp_hip_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().rvsp_hip_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceCsr:
+    """A CSR whose three arrays live in device memory."""
+
+    def __init__(self, host: Csr):
+        self.rows, self.cols, self.nnz = host.rows, host.cols, host.nnz
+        self.rp = DeviceBuffer.from_array(host.row_ptr)
+        self.ci = DeviceBuffer.from_array(host.col_idx)
+        self.va = DeviceBuffer.from_array(host.values)
+
+    def struct(self) -> CsrStruct:
+        return CsrStruct(self.rows, self.cols, self.nnz, self.rp.ptr, self.ci.ptr, self.va.ptr,
+                         DTYPE_FP32, FORMAT_CSR, 0)
+
+
+def device_csr_to_host(c: CsrStruct) -> Csr:
+    L = lib()
+    rp = np.empty(c.rows + 1, np.int32)
+    check("d2h", L.rvsp_hip_memcpy_d2h(rp.ctypes.data, c.row_ptr, rp.nbytes))
+    ci = np.empty(c.nnz, np.int32)
+    va = np.empty(c.nnz, np.float32)
+    if c.nnz:
+        check("d2h", L.rvsp_hip_memcpy_d2h(ci.ctypes.data, c.col_idx, ci.nbytes))
+        check("d2h", L.rvsp_hip_memcpy_d2h(va.ctypes.data, c.values, va.nbytes))
+    return Csr(c.rows, c.cols, rp, ci, va)
+
+
+def spgemm_device(dA: DeviceCsr, dB: DeviceCsr, algo=ALGO_DEFAULT, keep=False):
+    """Device one-shot; returns the host copy (and the device struct if keep)."""
+    a, b = dA.struct(), dB.struct()
+    c = CsrStruct()
+    check("csr_device", lib().rvsp_spgemm_csr_device_algo(C.byref(a), C.byref(b), C.byref(c), algo))
+    host = device_csr_to_host(c)
+    if keep:
+        return host, c
+    lib().rvsp_csr_destroy(C.byref(c))
+    return host
+
+
+PHASES = ("analyse", "fill", "numeric")
+
+
+def set_timing(on: bool):
+    lib().rvsp_hip_set_timing(int(on))
+
+
+def reset_timing():
+    lib().rvsp_hip_reset_timing()
+
+
+def get_timing():
+    """{phase: (total_ms, calls)} since the last reset."""
+    ms = (C.c_double * 4)()
+    calls = (C.c_longlong * 4)()
+    check("get_timing", lib().rvsp_hip_get_timing(ms, calls, 4))
+    return {name: (ms[i], calls[i]) for i, name in enumerate(PHASES)}
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        check("event_create", lib().rvsp_hip_event_create(C.byref(e)))
+        self.e = e.value
+
+    def record(self, stream=None):
+        check("event_record", lib().rvsp_hip_event_record(self.e, stream))
+
+    def sync(self):
+        check("event_sync", lib().rvsp_hip_event_synchronize(self.e))
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = C.c_float(0)
+        check("elapsed", lib().rvsp_hip_event_elapsed_ms(self.e, later.e, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().rvsp_hip_event_destroy(self.e)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# synthetic workloads (librvsp_gen.so)
+# ---------------------------------------------------------------------------
+def gen():
+    global _gen
+    if _gen is None:
+        if not os.path.exists(GEN_PATH):
+            raise RuntimeError(f"{GEN_PATH} missing: build with __graft_entry__.build()")
+        g = C.CDLL(GEN_PATH)
+        P, I32, I64, U64, V, D = C.POINTER, C.c_int32, C.c_int64, C.c_uint64, C.c_void_p, C.c_double
+        outs = [P(V), P(V), P(V)]
+        g.rvsp_gen_random_rows.restype = I64
+        g.rvsp_gen_random_rows.argtypes = [I32, I32, I32, I32, U64, C.c_int] + outs
+        g.rvsp_gen_rmat.restype = I64
+        g.rvsp_gen_rmat.argtypes = [I32, I64, D, D, D, U64, C.c_int] + outs
+        g.rvsp_gen_stencil27.restype = I64
+        g.rvsp_gen_stencil27.argtypes = [I32, U64, C.c_int] + outs
+        g.rvsp_gen_zipf_rows.restype = I64
+        g.rvsp_gen_zipf_rows.argtypes = [I32, I32, D, D, I32, U64, C.c_int] + outs
+        g.rvsp_gen_banded.restype = I64
+        g.rvsp_gen_banded.argtypes = [I32, I32, I32, I32, I32, U64, C.c_int] + outs
+        g.rvsp_gen_fill_values.restype = None
+        g.rvsp_gen_fill_values.argtypes = [V, I64, U64, C.c_int]
+        g.rvsp_gen_loguniform_sizes.restype = None
+        g.rvsp_gen_loguniform_sizes.argtypes = [I32, I32, I32, U64, V]
+        g.rvsp_gen_free.restype = None
+        g.rvsp_gen_free.argtypes = [V]
+        _gen = g
+    return _gen
+
+
+def _take(rows, cols, nnz, rp, ci, va) -> Csr:
+    g = gen()
+    if nnz < 0:
+        raise MemoryError("generator allocation failed")
+    out = Csr(
+        rows,
+        cols,
+        np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_int32)), (rows + 1,)).copy(),
+        np.ctypeslib.as_array(C.cast(ci, C.POINTER(C.c_int32)), (max(nnz, 1),))[:nnz].copy(),
+        np.ctypeslib.as_array(C.cast(va, C.POINTER(C.c_float)), (max(nnz, 1),))[:nnz].copy(),
+    )
+    for p in (rp, ci, va):
+        g.rvsp_gen_free(p)
+    return out
+
+
+def _outs():
+    return C.c_void_p(), C.c_void_p(), C.c_void_p()
+
+
+def gen_random_rows(m, n, lo, hi, seed, normal=False) -> Csr:
+    rp, ci, va = _outs()
+    nnz = gen().rvsp_gen_random_rows(m, n, lo, hi, seed, int(normal), C.byref(rp), C.byref(ci), C.byref(va))
+    return _take(m, n, nnz, rp.value, ci.value, va.value)
+
+
+def gen_rmat(scale, edges, a, b, c, seed, normal=False) -> Csr:
+    rp, ci, va = _outs()
+    nnz = gen().rvsp_gen_rmat(scale, edges, a, b, c, seed, int(normal), C.byref(rp), C.byref(ci), C.byref(va))
+    n = 1 << scale
+    return _take(n, n, nnz, rp.value, ci.value, va.value)
+
+
+def gen_stencil27(g_, seed, normal=True) -> Csr:
+    rp, ci, va = _outs()
+    nnz = gen().rvsp_gen_stencil27(g_, seed, int(normal), C.byref(rp), C.byref(ci), C.byref(va))
+    n = g_ ** 3
+    return _take(n, n, nnz, rp.value, ci.value, va.value)
+
+
+def gen_zipf_rows(m, n, s, mean, maxlen, seed, normal=True) -> Csr:
+    rp, ci, va = _outs()
+    nnz = gen().rvsp_gen_zipf_rows(m, n, s, mean, maxlen, seed, int(normal), C.byref(rp), C.byref(ci), C.byref(va))
+    return _take(m, n, nnz, rp.value, ci.value, va.value)
+
+
+def gen_banded(m, n, per_row, stride, half, seed, normal=True) -> Csr:
+    rp, ci, va = _outs()
+    nnz = gen().rvsp_gen_banded(m, n, per_row, stride, half, seed, int(normal), C.byref(rp), C.byref(ci), C.byref(va))
+    return _take(m, n, nnz, rp.value, ci.value, va.value)
+
+
+def fill_values(count, seed, normal=False) -> np.ndarray:
+    out = np.empty(count, np.float32)
+    gen().rvsp_gen_fill_values(out.ctypes.data, count, seed, int(normal))
+    return out
+
+
+def loguniform_sizes(count, lo, hi, seed) -> np.ndarray:
+    out = np.empty(count, np.int32)
+    gen().rvsp_gen_loguniform_sizes(count, lo, hi, seed, out.ctypes.data)
+    return out

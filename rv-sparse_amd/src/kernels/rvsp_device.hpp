@@ -1,0 +1,158 @@
+// rvsp_device.hpp -- shared device helpers for the gfx950 SpGEMM kernels.
+//
+// Wave64 throughout. "Group" = G consecutive lanes that cooperate on one
+// output row (G divides 64, or G = 256 for a whole block).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace rvsp {
+
+constexpr int kWave = 64;
+
+// Error bits raised by kernels (read back by rvsp_hip_plan_check).
+constexpr int kErrTableFull = 1;    // hash insert found no free slot
+constexpr int kErrProbeMiss = 2;    // lookup of a column C's structure lacks
+constexpr int kErrWindow = 4;       // product column outside its window
+
+// Symbolic bins (per row, from product count and column span).
+enum SymBin : int {
+    SB_EMPTY = 0,
+    SB_H64, SB_H128, SB_H256, SB_H512, SB_H1024, SB_H2048, SB_H4096,
+    SB_BMS,  // one-pass bitmap over a span of <= 16384 columns
+    SB_BIG,  // multi-pass 2^19-column bitmap, plus window cursors
+    SB_COUNT
+};
+
+// Numeric bins: 0 empty, 1 + 4*(log2(H)-6) + (log2(G)-3) for LDS hash
+// tables of H slots worked by G lanes, then a single dense window over the
+// row span, then column windows with cursors.
+constexpr int NB_EMPTY = 0;
+constexpr int NB_HASH0 = 1;
+constexpr int NB_DENSE = 1 + 7 * 4;  // 29
+constexpr int NB_WIN = NB_DENSE + 1; // 30
+constexpr int NB_COUNT = NB_WIN + 1; // 31
+
+constexpr int kBmsWords = 512;            // 16384 columns
+constexpr int kBigWords = 16384;          // 2^19 columns per pass
+constexpr int kWinCols = 4096;            // numeric window width (16 KB acc)
+constexpr int kDenseCols = 16384;         // single dense window (64 KB acc)
+constexpr int kHashMaxNnz = 2048;         // numeric hash rows: nnz(C_i) <= this
+constexpr int kSymHashMaxU = 2048;        // symbolic hash rows: min(ops,span) <= this
+
+__device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
+
+// Orders LDS traffic of one wave: everything issued before is visible to
+// every lane of the wave after. Same sequence rocPRIM uses for wave-level
+// cooperation through LDS.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__host__ __device__ constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
+
+__device__ __forceinline__ uint32_t hash_slot(int32_t col, int lg)
+{
+    return ((uint32_t)col * 0x9E3779B1u) >> (32 - lg);
+}
+
+__device__ __forceinline__ int next_pow2(int v)
+{
+    return v <= 1 ? 1 : 1 << (32 - __clz(v - 1));
+}
+
+// Inserts col into an LDS key table of H slots (empty = -1). Returns 1 when
+// this call placed it, 0 when it was present. Linear probing, bounded.
+template <int H>
+__device__ __forceinline__ int hash_insert(int32_t *T, int32_t col, int *err)
+{
+    constexpr int LG = ilog2(H);
+    uint32_t s = hash_slot(col, LG);
+    for (int probe = 0; probe < H; ++probe) {
+        int32_t cur = __hip_atomic_load(&T[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == col) return 0;
+        if (cur == -1) {
+            int32_t old = atomicCAS(&T[s], -1, col);
+            if (old == -1) return 1;
+            if (old == col) return 0;
+        }
+        s = (s + 1) & (H - 1);
+    }
+    atomicOr(err, kErrTableFull);
+    return 0;
+}
+
+// Inserts col and returns its slot (-1 if the table is full).
+template <int H>
+__device__ __forceinline__ int hash_insert_slot(int32_t *T, int32_t col, int *err)
+{
+    constexpr int LG = ilog2(H);
+    uint32_t s = hash_slot(col, LG);
+    for (int probe = 0; probe < H; ++probe) {
+        int32_t cur = __hip_atomic_load(&T[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == col) return (int)s;
+        if (cur == -1) {
+            int32_t old = atomicCAS(&T[s], -1, col);
+            if (old == -1 || old == col) return (int)s;
+        }
+        s = (s + 1) & (H - 1);
+    }
+    atomicOr(err, kErrTableFull);
+    return -1;
+}
+
+// Slot of a key known to be present (-1 and an error bit if absent).
+template <int H>
+__device__ __forceinline__ int hash_find(const int32_t *T, int32_t col, int *err)
+{
+    constexpr int LG = ilog2(H);
+    uint32_t s = hash_slot(col, LG);
+    for (int probe = 0; probe < H; ++probe) {
+        int32_t cur = T[s];
+        if (cur == col) return (int)s;
+        if (cur == -1) break;
+        s = (s + 1) & (H - 1);
+    }
+    atomicOr(err, kErrProbeMiss);
+    return -1;
+}
+
+template <int G>
+__device__ __forceinline__ int group_sum(int v)
+{
+    static_assert(G <= kWave, "group_sum is wave-level");
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+    return v;
+}
+
+// First index p in [lo, hi) of a sorted device array with a[p] >= key,
+// found by the whole wave (uniform result). Every lane must call it.
+__device__ __forceinline__ int wave_lower_bound(const int32_t *__restrict__ a, int lo, int hi,
+                                                int32_t key)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    while (hi - lo > kWave) {
+        const int len = hi - lo;
+        // lane j samples the last element of the j-th of 64 slices
+        const int pos = lo + (int)(((int64_t)(lane + 1) * len) / kWave) - 1;
+        const bool below = a[pos] < key;
+        const uint64_t m = __ballot(below);
+        const int c = __popcll(m);  // slices entirely below key
+        const int nlo = lo + (int)(((int64_t)c * len) / kWave);
+        const int nhi = c == kWave ? hi : lo + (int)(((int64_t)(c + 1) * len) / kWave);
+        lo = nlo;
+        hi = nhi;
+    }
+    const int p = lo + lane;
+    const bool below = p < hi && a[p] < key;
+    return lo + __popcll(__ballot(below));
+}
+
+} // namespace rvsp

@@ -1,0 +1,217 @@
+// rvsp_numeric.hip -- numeric pass of C = A * B (FP32) on gfx950.
+//
+// Arithmetic contract (rv-sparse gustavson_core_f32.inc:31-63 with
+// accum_scalar_f32.c:67-70, built without FMA contraction): every C entry
+// starts at +0.0f and receives fl(a_ik * b_kj) added with one rounding, in
+// ascending A-row position (ascending k) order; C entries are drained in
+// C's stored column order. The kernels below keep exactly that order, so
+// the values are bit-identical to the scalar kernel:
+//   * one row (or one column window of a row) belongs to lanes of ONE wave;
+//   * the A entries of the row are walked in stored order, and for each
+//     the lanes cover B_k's entries, whose columns are distinct;
+//   * each contribution is one LDS float add, and a wave's LDS operations
+//     execute in issue order, so successive k-steps reach an accumulator
+//     slot in k order. A compiler fence between k-steps keeps the issue
+//     order equal to program order.
+// Rows are binned (exact nnz(C_i) from the symbolic pass):
+//   hash    nnz(C_i) <= 2048: LDS table of H >= 2 nnz slots, pre-loaded with
+//           C_i's columns, G lanes per row (G ~ mean |B_k|)
+//   dense   one dense LDS window over the row's span (<= 16384 columns)
+//   window  rows the symbolic pass found big: kWinCols-column windows, each
+//           a dense LDS accumulator owned by one wave, with per-(A entry,
+//           window) cursors into B_k written by the symbolic pass
+#include "rvsp_device.hpp"
+#include "rvsp_kernels.hpp"
+
+namespace rvsp {
+
+template <int G, int H, int R>
+__global__ __launch_bounds__(R *G) void k_num_hash(const int32_t *__restrict__ rows, int32_t nrows,
+                                                   NumArgs a)
+{
+    __shared__ int32_t keys[R * H];
+    __shared__ float vals[R * H];
+    const int g = threadIdx.x / G, lane = threadIdx.x % G;
+    const int64_t ridx = (int64_t)blockIdx.x * R + g;
+    const bool valid = ridx < nrows;
+    const int row = valid ? rows[ridx] : 0;
+    int32_t *T = keys + g * H;
+    float *V = vals + g * H;
+    for (int s = lane; s < H; s += G) {
+        T[s] = -1;
+        V[s] = 0.0f;
+    }
+    __syncthreads();
+    const int cs = valid ? a.crp[row] : 0, ce = valid ? a.crp[row + 1] : 0;
+    for (int q = cs + lane; q < ce; q += G) hash_insert_slot<H>(T, a.cci[q], a.err);
+    __syncthreads();
+    if (valid) {
+        const int as = a.arp[row], ae = a.arp[row + 1];
+        for (int base = as; base < ae; base += G) {
+            const int ap = base + lane;
+            float av = 0.0f;
+            int bs = 0, be = 0;
+            if (ap < ae) {
+                const int k = a.aci[ap];
+                av = a.aval[ap];
+                bs = a.brp[k];
+                be = a.brp[k + 1];
+            }
+            const int nch = min(G, ae - base);
+            for (int j = 0; j < nch; ++j) {
+                const float aj = __shfl(av, j, G);
+                const int sj = __shfl(bs, j, G), ej = __shfl(be, j, G);
+                for (int bp = sj + lane; bp < ej; bp += G) {
+                    const int32_t col = a.bci[bp];
+                    const float prod = __fmul_rn(aj, a.bval[bp]);
+                    const int s = hash_find<H>(T, col, a.err);
+                    if (s >= 0) atomicAdd(&V[s], prod);
+                }
+                compiler_fence();
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = cs + lane; q < ce; q += G) {
+        const int s = hash_find<H>(T, a.cci[q], a.err);
+        a.cval[q] = s >= 0 ? V[s] : 0.0f;
+    }
+}
+
+// One wave per window. CURSORS: items are (row, window) pairs and B_k's
+// slice for the window comes from the symbolic pass's cursors. Otherwise
+// items are rows whose whole span fits one window.
+template <int WCOLS, int NW, bool CURSORS>
+__global__ __launch_bounds__(NW * 64) void k_num_window(const int32_t *__restrict__ rows,
+                                                        const int2 *__restrict__ items,
+                                                        int32_t nitems, NumArgs a)
+{
+    __shared__ float acc[NW * WCOLS];
+    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int64_t it = (int64_t)blockIdx.x * NW + wv;
+    const bool valid = it < nitems;
+    int row = 0, win = 0;
+    if (valid) {
+        if constexpr (CURSORS) {
+            const int2 item = items[it];
+            row = item.x;
+            win = item.y;
+        } else {
+            row = rows[it];
+        }
+    }
+    float *S = acc + wv * WCOLS;
+    for (int i = lane; i < WCOLS; i += 64) S[i] = 0.0f;
+    const int32_t wlo = valid ? a.lo[row] + win * WCOLS : 0;
+    const int64_t whi64 = (int64_t)wlo + WCOLS;
+    const int32_t whi = whi64 > INT32_MAX ? INT32_MAX : (int32_t)whi64;
+    int p0 = 0, p1 = 0;
+    if (valid) {
+        const int cs = a.crp[row], ce = a.crp[row + 1];
+        p0 = wave_lower_bound(a.cci, cs, ce, wlo);
+        p1 = wave_lower_bound(a.cci, p0, ce, whi);
+    }
+    wave_sync();
+    if (valid && p1 > p0) {
+        const int as = a.arp[row], ae = a.arp[row + 1];
+        const int alen = ae - as;
+        const int64_t cbase = CURSORS ? a.cur_off[row] : 0;
+        for (int base = as; base < ae; base += 64) {
+            const int ap = base + lane;
+            float av = 0.0f;
+            int b0 = 0, b1 = 0;
+            if (ap < ae) {
+                av = a.aval[ap];
+                if constexpr (CURSORS) {
+                    b0 = a.cursors[cbase + (int64_t)win * alen + (ap - as)];
+                    b1 = a.cursors[cbase + (int64_t)(win + 1) * alen + (ap - as)];
+                } else {
+                    const int k = a.aci[ap];
+                    b0 = a.brp[k];
+                    b1 = a.brp[k + 1];
+                }
+            }
+            uint64_t live = __ballot(b1 > b0);
+            while (live) {
+                const int j = __ffsll((unsigned long long)live) - 1;
+                live &= live - 1;
+                const float aj = __shfl(av, j, 64);
+                const int sj = __shfl(b0, j, 64), ej = __shfl(b1, j, 64);
+                for (int bp = sj + lane; bp < ej; bp += 64) {
+                    const int32_t c = a.bci[bp] - wlo;
+                    const float prod = __fmul_rn(aj, a.bval[bp]);
+                    if ((uint32_t)c < (uint32_t)WCOLS)
+                        atomicAdd(&S[c], prod);
+                    else
+                        atomicOr(a.err, kErrWindow);
+                }
+                compiler_fence();
+            }
+        }
+    }
+    wave_sync();
+    for (int q = p0 + lane; q < p1; q += 64) a.cval[q] = S[a.cci[q] - wlo];
+}
+
+template <int G, int H>
+static void launch_hash(hipStream_t st, const NumArgs &a, const int32_t *rows, int nrows)
+{
+    constexpr int R = (256 / G) < (8192 / H) ? (256 / G) : (8192 / H);
+    static_assert(R * G >= 64, "block must hold at least one full wave");
+    const int grid = (nrows + R - 1) / R;
+    k_num_hash<G, H, R><<<grid, R * G, 0, st>>>(rows, nrows, a);
+}
+
+#define RVSP_HASH_CASE(HI, GI, H, G)                                                            \
+    case NB_HASH0 + 4 * (HI) + (GI):                                                            \
+        launch_hash<G, H>(st, a, rows, nr);                                                     \
+        break;
+
+void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
+                    const int32_t *off, const int2 *items, int nitems)
+{
+    for (int b = NB_HASH0; b < NB_COUNT; ++b) {
+        const int nr = count[b];
+        if (nr == 0) continue;
+        const int32_t *rows = perm + off[b];
+        switch (b) {
+            RVSP_HASH_CASE(0, 0, 64, 8)
+            RVSP_HASH_CASE(0, 1, 64, 16)
+            RVSP_HASH_CASE(0, 2, 64, 32)
+            RVSP_HASH_CASE(0, 3, 64, 64)
+            RVSP_HASH_CASE(1, 0, 128, 8)
+            RVSP_HASH_CASE(1, 1, 128, 16)
+            RVSP_HASH_CASE(1, 2, 128, 32)
+            RVSP_HASH_CASE(1, 3, 128, 64)
+            RVSP_HASH_CASE(2, 0, 256, 8)
+            RVSP_HASH_CASE(2, 1, 256, 16)
+            RVSP_HASH_CASE(2, 2, 256, 32)
+            RVSP_HASH_CASE(2, 3, 256, 64)
+            RVSP_HASH_CASE(3, 0, 512, 8)
+            RVSP_HASH_CASE(3, 1, 512, 16)
+            RVSP_HASH_CASE(3, 2, 512, 32)
+            RVSP_HASH_CASE(3, 3, 512, 64)
+            RVSP_HASH_CASE(4, 0, 1024, 8)
+            RVSP_HASH_CASE(4, 1, 1024, 16)
+            RVSP_HASH_CASE(4, 2, 1024, 32)
+            RVSP_HASH_CASE(4, 3, 1024, 64)
+            RVSP_HASH_CASE(5, 1, 2048, 16)
+            RVSP_HASH_CASE(5, 2, 2048, 32)
+            RVSP_HASH_CASE(5, 3, 2048, 64)
+            RVSP_HASH_CASE(6, 2, 4096, 32)
+            RVSP_HASH_CASE(6, 3, 4096, 64)
+        case NB_DENSE:
+            k_num_window<kDenseCols, 1, false><<<nr, 64, 0, st>>>(rows, nullptr, nr, a);
+            break;
+        default:
+            break;
+        }
+    }
+    if (nitems > 0) {
+        constexpr int NW = 4;
+        k_num_window<kWinCols, NW, true><<<(nitems + NW - 1) / NW, NW * 64, 0, st>>>(
+            nullptr, items, nitems, a);
+    }
+}
+
+} // namespace rvsp

@@ -1,0 +1,603 @@
+// rvsp_plan.hip -- the device half of a SpGEMM structure (the "plan") and
+// the C ABI the host core calls (rvsp_hip_abi.h), plus the public device
+// and timing helpers of rv_sparse_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "rv_sparse_hip.h"
+#include "rvsp_device.hpp"
+#include "rvsp_hip_abi.h"
+#include "rvsp_kernels.hpp"
+
+using namespace rvsp;
+
+namespace {
+
+int g_fail_allocs = 0;  // test hook: number of allocations to fail
+
+rvsp_status_t map_err(hipError_t e)
+{
+    switch (e) {
+    case hipSuccess: return RVSP_SUCCESS;
+    case hipErrorOutOfMemory: return RVSP_ERROR_ALLOCATION_FAILED;
+    case hipErrorInvalidValue: return RVSP_ERROR_INVALID_ARGUMENT;
+    default: return RVSP_ERROR_UNSUPPORTED_BACKEND;
+    }
+}
+
+#define HIP_TRY(expr)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            if (getenv("RVSP_HIP_VERBOSE"))                                                       \
+                fprintf(stderr, "rvsp: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(e_),   \
+                        __FILE__, __LINE__);                                                      \
+            return map_err(e_);                                                                   \
+        }                                                                                         \
+    } while (0)
+
+bool g_pool_ready = false;
+
+rvsp_status_t device_ready()
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return RVSP_ERROR_UNSUPPORTED_BACKEND;
+    if (!g_pool_ready) {
+        // keep freed stream-ordered allocations cached in the pool, so
+        // repeated products reuse memory instead of returning it to the OS
+        int dev = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t thr = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
+        g_pool_ready = true;
+    }
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t dalloc(void **p, size_t bytes, hipStream_t st)
+{
+    *p = nullptr;
+    if (g_fail_allocs > 0) {
+        --g_fail_allocs;
+        return RVSP_ERROR_ALLOCATION_FAILED;
+    }
+    if (bytes == 0) bytes = 256;
+    HIP_TRY(hipMallocAsync(p, bytes, st));
+    return RVSP_SUCCESS;
+}
+
+void dfree(void *p, hipStream_t st)
+{
+    if (p) (void)hipFreeAsync(p, st);
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Phase timing: when on, every analyse / fill / numeric call records a pair
+// of HIP events on its own stream; rvsp_hip_get_timing resolves them.
+enum Phase { PH_ANALYSE = 0, PH_FILL = 1, PH_NUMERIC = 2, PH_COUNT = 4 };
+struct PendingTime {
+    int phase;
+    hipEvent_t a, b;
+};
+std::mutex g_tmu;
+bool g_timing = false;
+std::vector<PendingTime> g_pending;
+double g_ms[PH_COUNT];
+long long g_calls[PH_COUNT];
+
+struct PhaseScope {
+    hipStream_t st;
+    int phase;
+    hipEvent_t a = nullptr, b = nullptr;
+    PhaseScope(hipStream_t s, int ph) : st(s), phase(ph)
+    {
+        if (!g_timing) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            a = b = nullptr;
+            return;
+        }
+        (void)hipEventRecord(a, st);
+    }
+    ~PhaseScope()
+    {
+        if (!a) return;
+        (void)hipEventRecord(b, st);
+        std::lock_guard<std::mutex> lk(g_tmu);
+        g_pending.push_back({phase, a, b});
+    }
+};
+
+} // namespace
+
+struct rvsp_hip_plan {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int32_t m = 0, k = 0, n = 0, nnz_a = 0, nnz_b = 0;
+    const int32_t *arp = nullptr, *aci = nullptr, *brp = nullptr, *bci = nullptr;
+
+    void *rowblock = nullptr;  // one allocation carved into the per-row arrays
+    int64_t *ops = nullptr;
+    int32_t *lo = nullptr, *hi = nullptr;
+    uint8_t *sbin = nullptr, *nbin = nullptr;
+    int32_t *cnt = nullptr, *crp = nullptr, *sperm = nullptr, *nperm = nullptr;
+    int64_t *tsum = nullptr;
+    AnalysisCounters *ctr = nullptr;
+    int32_t *binoff = nullptr;  // [0,32) offsets, [32,64) cursors
+    int *err = nullptr;
+
+    int64_t *cur_off = nullptr;
+    int32_t *cursors = nullptr;
+    int2 *items = nullptr;
+    int nitems = 0;
+
+    AnalysisCounters hctr;
+    int32_t sym_count[SB_COUNT], sym_off[SB_COUNT];
+    int32_t num_count[NB_COUNT], num_off[NB_COUNT];
+    int64_t nnzc = 0, products = 0;
+    bool analysed = false;
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const void *gkey[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+static void plan_release(rvsp_hip_plan *p)
+{
+    dfree(p->rowblock, p->stream);
+    dfree(p->cur_off, p->stream);
+    dfree(p->cursors, p->stream);
+    dfree(p->items, p->stream);
+    p->rowblock = nullptr;
+    p->cur_off = nullptr;
+    p->cursors = nullptr;
+    p->items = nullptr;
+    p->nitems = 0;
+    p->analysed = false;
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->gexec = nullptr;
+    p->graph = nullptr;
+    for (auto &k : p->gkey) k = nullptr;
+}
+
+extern "C" {
+
+rvsp_status_t rvsp_hip_plan_create(rvsp_hip_plan_t **out, void *stream)
+{
+    if (!out) return RVSP_ERROR_NULL_POINTER;
+    *out = nullptr;
+    rvsp_status_t st = device_ready();
+    if (st != RVSP_SUCCESS) return st;
+    rvsp_hip_plan *p = new (std::nothrow) rvsp_hip_plan();
+    if (!p) return RVSP_ERROR_ALLOCATION_FAILED;
+    if (stream) {
+        p->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete p;
+            return RVSP_ERROR_UNSUPPORTED_BACKEND;
+        }
+        p->own_stream = true;
+    }
+    *out = p;
+    return RVSP_SUCCESS;
+}
+
+void rvsp_hip_plan_destroy(rvsp_hip_plan_t *p)
+{
+    if (!p) return;
+    plan_release(p);
+    (void)hipStreamSynchronize(p->stream);
+    if (p->own_stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+void *rvsp_hip_plan_stream(const rvsp_hip_plan_t *p) { return p ? (void *)p->stream : nullptr; }
+
+const int32_t *rvsp_hip_plan_row_ptr(const rvsp_hip_plan_t *p) { return p ? p->crp : nullptr; }
+
+const int64_t *rvsp_hip_plan_op_counts(const rvsp_hip_plan_t *p) { return p ? p->ops : nullptr; }
+
+rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, int32_t n,
+                                    int32_t nnz_a, const int32_t *arp, const int32_t *aci,
+                                    int32_t nnz_b, const int32_t *brp, const int32_t *bci,
+                                    rvsp_select_t select, int64_t *c_nnz, int64_t *products)
+{
+    (void)select;
+    if (!p || !arp || !aci || !brp || !bci) return RVSP_ERROR_NULL_POINTER;
+    if (m <= 0 || k <= 0 || n <= 0 || nnz_a < 0 || nnz_b < 0) return RVSP_ERROR_INVALID_ARGUMENT;
+    hipStream_t st = p->stream;
+    plan_release(p);
+    p->m = m;
+    p->k = k;
+    p->n = n;
+    p->nnz_a = nnz_a;
+    p->nnz_b = nnz_b;
+    p->arp = arp;
+    p->aci = aci;
+    p->brp = brp;
+    p->bci = bci;
+
+    // ---- one carved allocation for everything sized by m
+    const size_t M = (size_t)m;
+    const int ntiles = scan_tiles(m);
+    size_t off = 0;
+    const size_t o_ops = off; off = align256(off + M * 8);
+    const size_t o_lo = off; off = align256(off + M * 4);
+    const size_t o_hi = off; off = align256(off + M * 4);
+    const size_t o_sbin = off; off = align256(off + M);
+    const size_t o_nbin = off; off = align256(off + M);
+    const size_t o_cnt = off; off = align256(off + M * 4);
+    const size_t o_crp = off; off = align256(off + (M + 1) * 4);
+    const size_t o_sperm = off; off = align256(off + M * 4);
+    const size_t o_nperm = off; off = align256(off + M * 4);
+    const size_t o_tsum = off; off = align256(off + (size_t)ntiles * 8);
+    const size_t o_ctr = off; off = align256(off + sizeof(AnalysisCounters));
+    const size_t o_binoff = off; off = align256(off + 64 * 4);
+    const size_t o_err = off; off = align256(off + 4);
+    rvsp_status_t s = dalloc(&p->rowblock, off, st);
+    if (s != RVSP_SUCCESS) return s;
+    char *base = (char *)p->rowblock;
+    p->ops = (int64_t *)(base + o_ops);
+    p->lo = (int32_t *)(base + o_lo);
+    p->hi = (int32_t *)(base + o_hi);
+    p->sbin = (uint8_t *)(base + o_sbin);
+    p->nbin = (uint8_t *)(base + o_nbin);
+    p->cnt = (int32_t *)(base + o_cnt);
+    p->crp = (int32_t *)(base + o_crp);
+    p->sperm = (int32_t *)(base + o_sperm);
+    p->nperm = (int32_t *)(base + o_nperm);
+    p->tsum = (int64_t *)(base + o_tsum);
+    p->ctr = (AnalysisCounters *)(base + o_ctr);
+    p->binoff = (int32_t *)(base + o_binoff);
+    p->err = (int *)(base + o_err);
+
+    PhaseScope timer(st, PH_ANALYSE);
+    HIP_TRY(hipMemsetAsync(p->ctr, 0, sizeof(AnalysisCounters), st));
+    HIP_TRY(hipMemsetAsync(p->binoff, 0, 64 * 4, st));
+    HIP_TRY(hipMemsetAsync(p->err, 0, 4, st));
+
+    // ---- 1. per-row product counts, spans and symbolic bins
+    launch_row_stats(st, m, arp, aci, brp, bci, p->ops, p->lo, p->hi, p->sbin, p->ctr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(&p->hctr, p->ctr, sizeof(AnalysisCounters), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    p->products = (int64_t)p->hctr.products;
+
+    int32_t hoff[64] = {0};
+    {
+        int run = 0;
+        for (int b = 0; b < SB_COUNT; ++b) {
+            p->sym_count[b] = p->hctr.sym_hist[b];
+            p->sym_off[b] = run;
+            hoff[b] = run;
+            run += p->sym_count[b];
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(p->binoff, hoff, 64 * 4, hipMemcpyHostToDevice, st));
+    launch_bin_scatter(st, m, p->sbin, p->binoff, p->binoff + 32, p->sperm, SB_COUNT);
+    HIP_TRY(hipGetLastError());
+
+    if (p->sym_count[SB_BIG] > 0) {
+        s = dalloc((void **)&p->cur_off, M * 8, st);
+        if (s == RVSP_SUCCESS)
+            s = dalloc((void **)&p->cursors, (size_t)p->hctr.big_cursor_cap * 4, st);
+        if (s == RVSP_SUCCESS) s = dalloc((void **)&p->items, (size_t)p->hctr.big_windows * 8, st);
+        if (s != RVSP_SUCCESS) {
+            plan_release(p);
+            return s;
+        }
+    }
+
+    // ---- 2. exact nnz(C_i)
+    SymArgs sa{arp, aci, brp, bci, p->lo, p->hi, p->cnt, p->crp, nullptr,
+               p->cur_off, p->cursors, &p->ctr->cursor_fill, p->err};
+    // rows without products get no kernel, so their count must start at 0
+    HIP_TRY(hipMemsetAsync(p->cnt, 0, M * 4, st));
+    launch_sym_count(st, sa, p->sperm, p->sym_count, p->sym_off);
+    HIP_TRY(hipGetLastError());
+    // ---- 3. row pointer
+    launch_scan(st, m, p->cnt, p->tsum, (int64_t *)&p->ctr->nnz_total, p->crp);
+    HIP_TRY(hipGetLastError());
+    // ---- 4. numeric bins
+    launch_num_bins(st, m, p->crp, p->ops, arp, p->sbin, p->lo, p->hi, p->nbin, p->items, p->ctr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(&p->hctr, p->ctr, sizeof(AnalysisCounters), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    p->nnzc = p->hctr.nnz_total;
+    p->nitems = p->hctr.nitems;
+    {
+        int run = 0;
+        memset(hoff, 0, sizeof(hoff));
+        for (int b = 0; b < NB_COUNT; ++b) {
+            p->num_count[b] = p->hctr.num_hist[b];
+            p->num_off[b] = run;
+            hoff[b] = run;
+            run += p->num_count[b];
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(p->binoff, hoff, 64 * 4, hipMemcpyHostToDevice, st));
+    launch_bin_scatter(st, m, p->nbin, p->binoff, p->binoff + 32, p->nperm, NB_COUNT);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    int herr = 0;
+    HIP_TRY(hipMemcpy(&herr, p->err, 4, hipMemcpyDeviceToHost));
+    if (herr) return RVSP_ERROR_INVALID_CSR;
+    if (c_nnz) *c_nnz = p->nnzc;
+    if (products) *products = p->products;
+    if (p->nnzc > INT32_MAX) return RVSP_ERROR_ALLOCATION_FAILED;
+    p->analysed = true;
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_plan_fill(rvsp_hip_plan_t *p, int32_t *cci)
+{
+    if (!p || !cci) return RVSP_ERROR_NULL_POINTER;
+    if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
+    SymArgs sa{p->arp, p->aci, p->brp, p->bci, p->lo, p->hi, p->cnt, p->crp, cci,
+               p->cur_off, p->cursors, &p->ctr->cursor_fill, p->err};
+    PhaseScope timer(p->stream, PH_FILL);
+    launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off);
+    HIP_TRY(hipGetLastError());
+    return RVSP_SUCCESS;
+}
+
+static NumArgs num_args(rvsp_hip_plan *p, const float *aval, const float *bval, const int32_t *cci,
+                        float *cval)
+{
+    return NumArgs{p->arp, p->aci, aval,    p->brp, p->bci,    bval,    p->crp,
+                   cci,    cval,   p->lo,   p->cur_off, p->cursors, p->err};
+}
+
+rvsp_status_t rvsp_hip_plan_numeric(rvsp_hip_plan_t *p, const float *aval, const float *bval,
+                                    const int32_t *cci, float *cval)
+{
+    if (!p || !aval || !bval || !cci || !cval) return RVSP_ERROR_NULL_POINTER;
+    if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
+    PhaseScope timer(p->stream, PH_NUMERIC);
+    launch_numeric(p->stream, num_args(p, aval, bval, cci, cval), p->nperm, p->num_count,
+                   p->num_off, p->items, p->nitems);
+    HIP_TRY(hipGetLastError());
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_plan_numeric_graph(rvsp_hip_plan_t *p, const float *aval,
+                                          const float *bval, const int32_t *cci, float *cval)
+{
+    if (!p || !aval || !bval || !cci || !cval) return RVSP_ERROR_NULL_POINTER;
+    if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
+    const void *key[4] = {aval, bval, cci, cval};
+    if (p->gexec && memcmp(key, p->gkey, sizeof(key)) != 0) {
+        (void)hipGraphExecDestroy(p->gexec);
+        (void)hipGraphDestroy(p->graph);
+        p->gexec = nullptr;
+        p->graph = nullptr;
+    }
+    if (!p->gexec) {
+        HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+        launch_numeric(p->stream, num_args(p, aval, bval, cci, cval), p->nperm, p->num_count,
+                       p->num_off, p->items, p->nitems);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(p->stream, &g);
+        if (e != hipSuccess) return map_err(e);
+        e = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return map_err(e);
+        }
+        p->graph = g;
+        memcpy(p->gkey, key, sizeof(key));
+    }
+    PhaseScope timer(p->stream, PH_NUMERIC);
+    HIP_TRY(hipGraphLaunch(p->gexec, p->stream));
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_plan_check(rvsp_hip_plan_t *p)
+{
+    if (!p) return RVSP_ERROR_NULL_POINTER;
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    if (!p->err) return RVSP_SUCCESS;
+    int herr = 0;
+    HIP_TRY(hipMemcpy(&herr, p->err, 4, hipMemcpyDeviceToHost));
+    return herr ? RVSP_ERROR_INVALID_CSR : RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_stream_sync(void *stream)
+{
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_copy_async(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (bytes == 0) return RVSP_SUCCESS;
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, (hipStream_t)stream));
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_alloc_async(void **ptr, size_t bytes, void *stream)
+{
+    if (!ptr) return RVSP_ERROR_NULL_POINTER;
+    rvsp_status_t s = device_ready();
+    if (s != RVSP_SUCCESS) return s;
+    return dalloc(ptr, bytes, (hipStream_t)stream);
+}
+
+rvsp_status_t rvsp_hip_free_async(void *ptr, void *stream)
+{
+    if (ptr) HIP_TRY(hipFreeAsync(ptr, (hipStream_t)stream));
+    return RVSP_SUCCESS;
+}
+
+void rvsp_hip_fail_next_allocs(int n) { g_fail_allocs = n; }
+
+void rvsp_hip_set_timing(int on)
+{
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_timing = on != 0;
+}
+
+rvsp_status_t rvsp_hip_get_timing(double *ms, long long *calls, int n)
+{
+    if (!ms || !calls) return RVSP_ERROR_NULL_POINTER;
+    std::vector<PendingTime> todo;
+    {
+        std::lock_guard<std::mutex> lk(g_tmu);
+        todo.swap(g_pending);
+    }
+    for (auto &t : todo) {
+        float e = 0.0f;
+        if (hipEventSynchronize(t.b) == hipSuccess && hipEventElapsedTime(&e, t.a, t.b) == hipSuccess) {
+            g_ms[t.phase] += e;
+            g_calls[t.phase] += 1;
+        }
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (int i = 0; i < n && i < PH_COUNT; ++i) {
+        ms[i] = g_ms[i];
+        calls[i] = g_calls[i];
+    }
+    return RVSP_SUCCESS;
+}
+
+void rvsp_hip_reset_timing(void)
+{
+    double ms[PH_COUNT];
+    long long c[PH_COUNT];
+    (void)rvsp_hip_get_timing(ms, c, PH_COUNT);
+    for (int i = 0; i < PH_COUNT; ++i) {
+        g_ms[i] = 0.0;
+        g_calls[i] = 0;
+    }
+}
+
+// ---- public helpers (rv_sparse_hip.h) ------------------------------------
+
+rvsp_status_t rvsp_hip_available(int *count)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    if (count) *count = c;
+    return c > 0 ? RVSP_SUCCESS : RVSP_ERROR_UNSUPPORTED_BACKEND;
+}
+
+rvsp_status_t rvsp_hip_malloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return RVSP_ERROR_NULL_POINTER;
+    *ptr = nullptr;
+    rvsp_status_t s = device_ready();
+    if (s != RVSP_SUCCESS) return s;
+    if (g_fail_allocs > 0) {
+        --g_fail_allocs;
+        return RVSP_ERROR_ALLOCATION_FAILED;
+    }
+    HIP_TRY(hipMalloc(ptr, bytes ? bytes : 256));
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_free(void *ptr)
+{
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return RVSP_SUCCESS;
+    if (!dst || !src) return RVSP_ERROR_NULL_POINTER;
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return RVSP_SUCCESS;
+    if (!dst || !src) return RVSP_ERROR_NULL_POINTER;
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_memcpy_d2d(void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return RVSP_SUCCESS;
+    if (!dst || !src) return RVSP_ERROR_NULL_POINTER;
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_memset(void *dst, int value, size_t bytes)
+{
+    if (bytes == 0) return RVSP_SUCCESS;
+    if (!dst) return RVSP_ERROR_NULL_POINTER;
+    HIP_TRY(hipMemset(dst, value, bytes));
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_synchronize(void)
+{
+    HIP_TRY(hipDeviceSynchronize());
+    return RVSP_SUCCESS;
+}
+
+struct rvsp_hip_event {
+    hipEvent_t ev;
+};
+
+rvsp_status_t rvsp_hip_event_create(rvsp_hip_event_t *ev)
+{
+    if (!ev) return RVSP_ERROR_NULL_POINTER;
+    *ev = nullptr;
+    rvsp_status_t s = device_ready();
+    if (s != RVSP_SUCCESS) return s;
+    rvsp_hip_event *e = new (std::nothrow) rvsp_hip_event();
+    if (!e) return RVSP_ERROR_ALLOCATION_FAILED;
+    if (hipEventCreate(&e->ev) != hipSuccess) {
+        delete e;
+        return RVSP_ERROR_UNSUPPORTED_BACKEND;
+    }
+    *ev = e;
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_event_record(rvsp_hip_event_t ev, void *stream)
+{
+    if (!ev) return RVSP_ERROR_NULL_POINTER;
+    HIP_TRY(hipEventRecord(ev->ev, (hipStream_t)stream));
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_event_synchronize(rvsp_hip_event_t ev)
+{
+    if (!ev) return RVSP_ERROR_NULL_POINTER;
+    HIP_TRY(hipEventSynchronize(ev->ev));
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_event_elapsed_ms(rvsp_hip_event_t a, rvsp_hip_event_t b, float *ms)
+{
+    if (!a || !b || !ms) return RVSP_ERROR_NULL_POINTER;
+    HIP_TRY(hipEventElapsedTime(ms, a->ev, b->ev));
+    return RVSP_SUCCESS;
+}
+
+void rvsp_hip_event_destroy(rvsp_hip_event_t ev)
+{
+    if (!ev) return;
+    (void)hipEventDestroy(ev->ev);
+    delete ev;
+}
+
+} // extern "C"

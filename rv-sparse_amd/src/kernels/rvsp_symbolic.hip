@@ -1,0 +1,560 @@
+// rvsp_symbolic.hip -- row analysis, binning, symbolic count/fill and the
+// row-pointer scan for C = A * B on gfx950.
+//
+// Follows the structure semantics of rv-sparse's symbolic phase
+// (src/kernels/spgemm/rvsp_symbolic.c:13-138): C_i's columns are the
+// distinct columns of the B rows that A_i references, stored ascending;
+// op_i = sum over A_i of len(B_k) in int64; row_ptr is the exclusive scan.
+// The GPU builds the distinct set in LDS (hash table or bitmap, chosen per
+// row) instead of the reference's mark/touched arrays, and sorts in LDS
+// (bitonic) or emits in bitmap order, so the result is the same set in the
+// same (ascending) order.
+#include "rvsp_device.hpp"
+#include "rvsp_kernels.hpp"
+
+namespace rvsp {
+
+// ---------------------------------------------------------------------------
+// Per-row statistics: product count, column span of the products, bin.
+// 16 lanes per row; lanes stride over the row's A entries.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int sym_bin_of(int64_t ops, int32_t lo, int32_t hi)
+{
+    if (ops == 0) return SB_EMPTY;
+    const int64_t span = (int64_t)hi - lo + 1;
+    const int64_t u = ops < span ? ops : span;
+    int H = 64;
+    if (u <= kSymHashMaxU) H = max(64, next_pow2((int)(2 * u)));
+    if (span <= 32 * kBmsWords && (u > kSymHashMaxU || span <= 32 * (int64_t)H)) return SB_BMS;
+    if (u <= kSymHashMaxU) return SB_H64 + (ilog2(H) - 6);
+    return SB_BIG;
+}
+
+__global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__restrict__ arp,
+                                                   const int32_t *__restrict__ aci,
+                                                   const int32_t *__restrict__ brp,
+                                                   const int32_t *__restrict__ bci,
+                                                   int64_t *__restrict__ ops_out,
+                                                   int32_t *__restrict__ lo_out,
+                                                   int32_t *__restrict__ hi_out,
+                                                   uint8_t *__restrict__ sbin_out,
+                                                   AnalysisCounters *__restrict__ ctr)
+{
+    constexpr int G = 16;
+    __shared__ int hist[SB_COUNT];
+    if (threadIdx.x < SB_COUNT) hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x % G;
+    const int64_t row = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+    int64_t ops = 0;
+    int32_t lo = INT32_MAX, hi = -1;
+    if (row < m) {
+        const int as = arp[row], ae = arp[row + 1];
+        for (int ap = as + lane; ap < ae; ap += G) {
+            const int k = aci[ap];
+            const int bs = brp[k], be = brp[k + 1];
+            ops += be - bs;
+            if (be > bs) {
+                lo = min(lo, bci[bs]);
+                hi = max(hi, bci[be - 1]);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        ops += __shfl_xor(ops, off, G);
+        lo = min(lo, __shfl_xor(lo, off, G));
+        hi = max(hi, __shfl_xor(hi, off, G));
+    }
+    if (row < m && lane == 0) {
+        const int b = sym_bin_of(ops, lo, hi);
+        ops_out[row] = ops;
+        lo_out[row] = ops ? lo : 0;
+        hi_out[row] = ops ? hi : -1;
+        sbin_out[row] = (uint8_t)b;
+        atomicAdd(&hist[b], 1);
+        if (b == SB_BIG) {
+            const int alen = arp[row + 1] - arp[row];
+            const int nwin = (int)(((int64_t)hi - lo + kWinCols) / kWinCols);
+            atomicAdd((unsigned long long *)&ctr->big_cursor_cap,
+                      (unsigned long long)alen * (unsigned long long)(nwin + 1));
+            atomicAdd((unsigned long long *)&ctr->big_windows, (unsigned long long)nwin);
+        }
+        atomicAdd((unsigned long long *)&ctr->products, (unsigned long long)ops);
+    }
+    __syncthreads();
+    if (threadIdx.x < SB_COUNT && hist[threadIdx.x])
+        atomicAdd(&ctr->sym_hist[threadIdx.x], hist[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// Stable-enough scatter of rows into per-bin lists (order inside a bin is
+// arbitrary; every row is processed independently, so results do not
+// depend on it).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bin_scatter(int32_t m, const uint8_t *__restrict__ bin,
+                                                     const int32_t *__restrict__ off,
+                                                     int32_t *__restrict__ cursor,
+                                                     int32_t *__restrict__ perm, int nb)
+{
+    __shared__ int lcount[64];
+    __shared__ int lbase[64];
+    if (threadIdx.x < 64) lcount[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int b = i < m ? (int)bin[i] : -1;
+    const int lpos = b >= 0 ? atomicAdd(&lcount[b], 1) : 0;
+    __syncthreads();
+    if (threadIdx.x < nb && lcount[threadIdx.x] > 0)
+        lbase[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], lcount[threadIdx.x]);
+    __syncthreads();
+    if (b >= 0) perm[off[b] + lbase[b] + lpos] = (int32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// Symbolic pass, hash variant: G lanes per row, LDS key table of H slots.
+// Count mode writes nnz(C_i); fill mode writes C_i's columns ascending
+// (compaction, then a bitonic sort of H/2 keys padded with INT32_MAX).
+// Product order does not matter for the structure, so lanes take B entries
+// freely.
+// ---------------------------------------------------------------------------
+template <int G, int H, bool FILL>
+__global__ __launch_bounds__(256) void k_sym_hash(const int32_t *__restrict__ rows, int32_t nrows,
+                                                  const int32_t *__restrict__ arp,
+                                                  const int32_t *__restrict__ aci,
+                                                  const int32_t *__restrict__ brp,
+                                                  const int32_t *__restrict__ bci,
+                                                  int32_t *__restrict__ cnt,
+                                                  const int32_t *__restrict__ crp,
+                                                  int32_t *__restrict__ cci, int *err)
+{
+    constexpr int R = 256 / G;
+    constexpr int P = H / 2;  // capacity of the sorted list (nnz <= H/2)
+    __shared__ int32_t keys[R * H];
+    __shared__ int32_t list[FILL ? R * P : 1];
+    __shared__ int counts[R];
+    const int g = threadIdx.x / G, lane = threadIdx.x % G;
+    const int64_t ridx = (int64_t)blockIdx.x * R + g;
+    const bool valid = ridx < nrows;
+    const int row = valid ? rows[ridx] : 0;
+    int32_t *T = keys + g * H;
+    for (int s = lane; s < H; s += G) T[s] = -1;
+    if (lane == 0) counts[g] = 0;
+    __syncthreads();
+    int mine = 0;
+    if (valid) {
+        const int as = arp[row], ae = arp[row + 1];
+        for (int ap = as; ap < ae; ++ap) {
+            const int k = aci[ap];
+            const int bs = brp[k], be = brp[k + 1];
+            for (int bp = bs + lane; bp < be; bp += G) mine += hash_insert<H>(T, bci[bp], err);
+        }
+    }
+    if (mine) atomicAdd(&counts[g], mine);
+    __syncthreads();
+    if constexpr (!FILL) {
+        if (valid && lane == 0) cnt[row] = counts[g];
+        return;
+    } else {
+        int32_t *L = list + g * P;
+        const int total = counts[g];
+        __syncthreads();
+        if (lane == 0) counts[g] = 0;
+        for (int s = lane; s < P; s += G) L[s] = INT32_MAX;
+        __syncthreads();
+        for (int s = lane; s < H; s += G) {
+            const int32_t key = T[s];
+            if (key >= 0) {
+                const int pos = atomicAdd(&counts[g], 1);
+                if (pos < P) L[pos] = key;
+            }
+        }
+        __syncthreads();
+        // bitonic sort of P keys, ascending; every group runs the same
+        // stage sequence, so block barriers are uniform.
+        for (int kk = 2; kk <= P; kk <<= 1) {
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+                for (int i = lane; i < P; i += G) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const int32_t x = L[i], y = L[ixj];
+                        const bool up = (i & kk) == 0;
+                        if ((x > y) == up) {
+                            L[i] = y;
+                            L[ixj] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (valid) {
+            const int out = crp[row];
+            for (int i = lane; i < total && i < P; i += G) cci[out + i] = L[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Symbolic pass, bitmap variant: one bit per column of a window of
+// 32*WORDS columns starting at the row's smallest product column. Small
+// rows (span <= 16384) take one pass; big rows loop windows of 2^19
+// columns. Fill mode emits set bits in ascending order, which is C_i's
+// column order by construction. In count mode big rows also record, for
+// every A entry and every numeric window of kWinCols columns, where that
+// window starts inside B_k (the numeric window kernel's cursors).
+// ---------------------------------------------------------------------------
+template <int G, int WORDS, bool FILL, bool BIG>
+__global__ __launch_bounds__(256) void k_sym_bitmap(const int32_t *__restrict__ rows, int32_t nrows,
+                                                    const int32_t *__restrict__ arp,
+                                                    const int32_t *__restrict__ aci,
+                                                    const int32_t *__restrict__ brp,
+                                                    const int32_t *__restrict__ bci,
+                                                    const int32_t *__restrict__ lo_col,
+                                                    const int32_t *__restrict__ hi_col,
+                                                    int32_t *__restrict__ cnt,
+                                                    const int32_t *__restrict__ crp,
+                                                    int32_t *__restrict__ cci,
+                                                    int64_t *__restrict__ cur_off,
+                                                    int32_t *__restrict__ cursors,
+                                                    unsigned long long *cursor_fill, int *err)
+{
+    constexpr int R = 256 / G;
+    constexpr int COLS = 32 * WORDS;
+    constexpr int WPT = WORDS / G;  // words per thread in the ordered scan
+    static_assert(WORDS % G == 0, "words per thread");
+    __shared__ uint32_t bits[R * WORDS];
+    __shared__ int scan[256];
+    __shared__ long long cbase_s[R];
+    const int g = threadIdx.x / G, lane = threadIdx.x % G;
+    const int64_t ridx = (int64_t)blockIdx.x * R + g;
+    const bool valid = ridx < nrows;
+    const int row = valid ? rows[ridx] : 0;
+    uint32_t *Bm = bits + g * WORDS;
+    const int32_t lo = valid ? lo_col[row] : 0;
+    const int32_t hi = valid ? hi_col[row] : -1;
+    const int as = valid ? arp[row] : 0, ae = valid ? arp[row + 1] : 0;
+    const int alen = ae - as;
+    int64_t cbase = 0;
+    int nwin = 0;
+    if constexpr (BIG && !FILL) {
+        nwin = (int)(((int64_t)hi - lo + kWinCols) / kWinCols);
+        if (valid && lane == 0) {
+            cbase = (long long)atomicAdd(cursor_fill, (unsigned long long)alen * (unsigned long long)(nwin + 1));
+            cur_off[row] = cbase;
+            cbase_s[g] = cbase;
+        }
+        __syncthreads();
+        cbase = cbase_s[g];
+    }
+    // BIG rows use one block per row (R == 1), so the pass count is uniform
+    // across the block; small rows always take a single pass.
+    const int npass = BIG ? (int)(((int64_t)hi - lo + COLS) / COLS) : 1;
+    int total = 0;
+    int out = (FILL && valid) ? crp[row] : 0;
+    for (int pass = 0; pass < npass; ++pass) {
+        const int32_t base = lo + pass * COLS;
+        for (int w = lane; w < WORDS; w += G) Bm[w] = 0u;
+        __syncthreads();
+        if (valid) {
+            for (int ap = as; ap < ae; ++ap) {
+                const int k = aci[ap];
+                const int bs = brp[k], be = brp[k + 1];
+                for (int bp = bs + lane; bp < be; bp += G) {
+                    const int32_t col = bci[bp];
+                    const int32_t c = col - base;
+                    if (c >= 0 && c < COLS) atomicOr(&Bm[c >> 5], 1u << (c & 31));
+                    if constexpr (BIG && !FILL) {
+                        if (pass == 0) {
+                            // window boundaries crossed between the previous
+                            // entry and this one start at bp
+                            const int wcur = (col - lo) / kWinCols;
+                            const int wprev = bp > bs ? (bci[bp - 1] - lo) / kWinCols : -1;
+                            for (int w = wprev + 1; w <= wcur; ++w)
+                                cursors[cbase + (int64_t)w * alen + (ap - as)] = bp;
+                            if (bp == be - 1)
+                                for (int w = wcur + 1; w <= nwin; ++w)
+                                    cursors[cbase + (int64_t)w * alen + (ap - as)] = be;
+                        }
+                    }
+                }
+                if constexpr (BIG && !FILL) {
+                    if (pass == 0 && be == bs && lane == 0)
+                        for (int w = 0; w <= nwin; ++w) cursors[cbase + (int64_t)w * alen + (ap - as)] = bs;
+                }
+            }
+        }
+        __syncthreads();
+        // ordered scan: thread `lane` owns words [lane*WPT, (lane+1)*WPT)
+        int local = 0;
+#pragma unroll 4
+        for (int w = 0; w < WPT; ++w) local += __popc(Bm[lane * WPT + w]);
+        scan[threadIdx.x] = local;
+        __syncthreads();
+        // exclusive prefix over the group (G <= 256, simple serial-in-LDS
+        // Hillis-Steele)
+        for (int off = 1; off < G; off <<= 1) {
+            const int v = lane >= off ? scan[threadIdx.x - off] : 0;
+            __syncthreads();
+            scan[threadIdx.x] += v;
+            __syncthreads();
+        }
+        const int incl = scan[threadIdx.x];
+        const int gtotal = scan[g * G + G - 1];
+        if constexpr (FILL) {
+            if (valid) {
+                int pos = out + incl - local;
+                for (int w = 0; w < WPT; ++w) {
+                    uint32_t word = Bm[lane * WPT + w];
+                    const int32_t cb = base + (lane * WPT + w) * 32;
+                    while (word) {
+                        const int b = __ffs(word) - 1;
+                        word &= word - 1;
+                        cci[pos++] = cb + b;
+                    }
+                }
+            }
+            out += gtotal;
+        }
+        total += gtotal;
+        __syncthreads();
+    }
+    if constexpr (!FILL) {
+        if (valid && lane == 0) cnt[row] = total;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of per-row counts into C's row pointer (int64 running sum,
+// int32 stored; the host rejects totals above INT32_MAX). Three launches:
+// tile sums, scan of tile sums, tile scan plus offset.
+// ---------------------------------------------------------------------------
+constexpr int kScanItems = 8;
+constexpr int kScanTile = 256 * kScanItems;
+
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t *total)
+{
+    __shared__ int64_t wsum[4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int64_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int64_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    int64_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        if (w < wid) before += wsum[w];
+        all += wsum[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tiles(int32_t m, const int32_t *__restrict__ cnt,
+                                                    int64_t *__restrict__ tsum)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int64_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j)
+        if (base + j < m) s += cnt[base + j];
+    int64_t tot;
+    block_excl_scan(s, &tot);
+    if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_scan_top(int32_t ntiles, int64_t *__restrict__ tsum,
+                                                  int64_t *__restrict__ total_out)
+{
+    int64_t carry = 0;
+    for (int base = 0; base < ntiles; base += 256) {
+        const int i = base + threadIdx.x;
+        const int64_t v = i < ntiles ? tsum[i] : 0;
+        int64_t tot;
+        const int64_t ex = block_excl_scan(v, &tot);
+        if (i < ntiles) tsum[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(int32_t m, const int32_t *__restrict__ cnt,
+                                                    const int64_t *__restrict__ tsum,
+                                                    const int64_t *__restrict__ total,
+                                                    int32_t *__restrict__ crp)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int32_t v[kScanItems];
+    int64_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        v[j] = base + j < m ? cnt[base + j] : 0;
+        s += v[j];
+    }
+    int64_t tot;
+    int64_t run = tsum[blockIdx.x] + block_excl_scan(s, &tot);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j)
+        if (base + j < m) {
+            crp[base + j] = (int32_t)run;
+            run += v[j];
+        }
+    if (blockIdx.x == 0 && threadIdx.x == 0) crp[m] = (int32_t)*total;
+}
+
+// ---------------------------------------------------------------------------
+// Numeric binning from exact nnz(C_i): LDS hash (H from nnz, G from the
+// mean B-row length), a single dense window, or column windows (rows the
+// symbolic pass treated as big, which have cursors).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int num_hash_bin(int nz, int64_t ops, int alen)
+{
+    const int H = max(64, next_pow2(2 * nz));
+    const int64_t avgb = alen > 0 ? (ops + alen - 1) / alen : 1;
+    int G = next_pow2((int)min<int64_t>(avgb, 64));
+    G = max(8, min(64, G));
+    const int rmax = 8192 / H;  // rows per block so that 8*H*R <= 64 KB
+    const int gmin = rmax >= 8 ? 8 : 64 / rmax;
+    G = max(G, gmin);
+    return NB_HASH0 + 4 * (ilog2(H) - 6) + (ilog2(G) - 3);
+}
+
+__global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__restrict__ crp,
+                                                  const int64_t *__restrict__ ops,
+                                                  const int32_t *__restrict__ arp,
+                                                  const uint8_t *__restrict__ sbin,
+                                                  const int32_t *__restrict__ lo_col,
+                                                  const int32_t *__restrict__ hi_col,
+                                                  uint8_t *__restrict__ nbin,
+                                                  int2 *__restrict__ items,
+                                                  AnalysisCounters *__restrict__ ctr)
+{
+    __shared__ int hist[NB_COUNT];
+    if (threadIdx.x < NB_COUNT) hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < m) {
+        const int nz = crp[i + 1] - crp[i];
+        int b;
+        if (nz == 0) {
+            b = NB_EMPTY;
+        } else if (sbin[i] == SB_BIG) {
+            b = NB_WIN;
+            const int nwin = (int)(((int64_t)hi_col[i] - lo_col[i] + kWinCols) / kWinCols);
+            const int base = atomicAdd(&ctr->nitems, nwin);
+            for (int w = 0; w < nwin; ++w) items[base + w] = make_int2((int)i, w);
+        } else if (nz <= kHashMaxNnz) {
+            b = num_hash_bin(nz, ops[i], arp[i + 1] - arp[i]);
+        } else {
+            b = NB_DENSE;
+        }
+        nbin[i] = (uint8_t)b;
+        atomicAdd(&hist[b], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < NB_COUNT && hist[threadIdx.x])
+        atomicAdd(&ctr->num_hist[threadIdx.x], hist[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+template <int G, int H, bool FILL>
+static void launch_sym_hash(hipStream_t st, const SymArgs &a, const int32_t *rows, int nrows)
+{
+    const int R = 256 / G;
+    const int grid = (nrows + R - 1) / R;
+    k_sym_hash<G, H, FILL><<<grid, 256, 0, st>>>(rows, nrows, a.arp, a.aci, a.brp, a.bci, a.cnt,
+                                                  a.crp, a.cci, a.err);
+}
+
+template <bool FILL>
+static void launch_sym_bins(hipStream_t st, const SymArgs &a, const int32_t *perm,
+                            const int32_t *count, const int32_t *off)
+{
+    for (int b = SB_H64; b < SB_COUNT; ++b) {
+        const int nr = count[b];
+        if (nr == 0) continue;
+        const int32_t *rows = perm + off[b];
+        switch (b) {
+        case SB_H64: launch_sym_hash<8, 64, FILL>(st, a, rows, nr); break;
+        case SB_H128: launch_sym_hash<16, 128, FILL>(st, a, rows, nr); break;
+        case SB_H256: launch_sym_hash<32, 256, FILL>(st, a, rows, nr); break;
+        case SB_H512: launch_sym_hash<64, 512, FILL>(st, a, rows, nr); break;
+        case SB_H1024: launch_sym_hash<64, 1024, FILL>(st, a, rows, nr); break;
+        case SB_H2048: launch_sym_hash<128, 2048, FILL>(st, a, rows, nr); break;
+        case SB_H4096: launch_sym_hash<256, 4096, FILL>(st, a, rows, nr); break;
+        case SB_BMS: {
+            const int grid = (nr + 3) / 4;
+            k_sym_bitmap<64, kBmsWords, FILL, false><<<grid, 256, 0, st>>>(
+                rows, nr, a.arp, a.aci, a.brp, a.bci, a.lo, a.hi, a.cnt, a.crp, a.cci, a.cur_off,
+                a.cursors, a.cursor_fill, a.err);
+            break;
+        }
+        case SB_BIG:
+            k_sym_bitmap<256, kBigWords, FILL, true><<<nr, 256, 0, st>>>(
+                rows, nr, a.arp, a.aci, a.brp, a.bci, a.lo, a.hi, a.cnt, a.crp, a.cci, a.cur_off,
+                a.cursors, a.cursor_fill, a.err);
+            break;
+        default: break;
+        }
+    }
+}
+
+void launch_row_stats(hipStream_t st, int32_t m, const int32_t *arp, const int32_t *aci,
+                      const int32_t *brp, const int32_t *bci, int64_t *ops, int32_t *lo,
+                      int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr)
+{
+    const int grid = (int)(((int64_t)m + 15) / 16);
+    k_row_stats<<<grid, 256, 0, st>>>(m, arp, aci, brp, bci, ops, lo, hi, sbin, ctr);
+}
+
+void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int32_t *off,
+                        int32_t *cursor, int32_t *perm, int nb)
+{
+    const int grid = (int)(((int64_t)m + 255) / 256);
+    k_bin_scatter<<<grid, 256, 0, st>>>(m, bin, off, cursor, perm, nb);
+}
+
+void launch_sym_count(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
+                      const int32_t *off)
+{
+    launch_sym_bins<false>(st, a, perm, count, off);
+}
+
+void launch_sym_fill(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
+                     const int32_t *off)
+{
+    launch_sym_bins<true>(st, a, perm, count, off);
+}
+
+void launch_scan(hipStream_t st, int32_t m, const int32_t *cnt, int64_t *tsum, int64_t *total,
+                 int32_t *crp)
+{
+    const int ntiles = (int)(((int64_t)m + kScanTile - 1) / kScanTile);
+    const int nt = ntiles > 0 ? ntiles : 1;
+    k_scan_tiles<<<nt, 256, 0, st>>>(m, cnt, tsum);
+    k_scan_top<<<1, 256, 0, st>>>(nt, tsum, total);
+    k_scan_apply<<<nt, 256, 0, st>>>(m, cnt, tsum, total, crp);
+}
+
+int scan_tiles(int32_t m)
+{
+    const int ntiles = (int)(((int64_t)m + kScanTile - 1) / kScanTile);
+    return ntiles > 0 ? ntiles : 1;
+}
+
+void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_t *ops,
+                     const int32_t *arp, const uint8_t *sbin, const int32_t *lo,
+                     const int32_t *hi, uint8_t *nbin, int2 *items, AnalysisCounters *ctr)
+{
+    const int grid = (int)(((int64_t)m + 255) / 256);
+    k_num_bins<<<grid, 256, 0, st>>>(m, crp, ops, arp, sbin, lo, hi, nbin, items, ctr);
+}
+
+} // namespace rvsp
