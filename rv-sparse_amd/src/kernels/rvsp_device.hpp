@@ -37,8 +37,12 @@ constexpr int NB_WIN = NB_DENSE + 1; // 30
 constexpr int NB_COUNT = NB_WIN + 1; // 31
 
 constexpr int kBmsWords = 512;            // 16384 columns
-constexpr int kBigWords = 16384;          // 2^19 columns per pass
-constexpr int kWinCols = 4096;            // numeric window width (16 KB acc)
+constexpr int kBigPassWords = 16384;      // big-row bitmap: 2^19 columns per pass
+constexpr int kChunkCols = 1024;          // big-row chunk (window granularity)
+constexpr int kChunkWords = kChunkCols / 32;
+constexpr int kBigPW = 2048;              // outputs per big-row window
+constexpr int kBigMaxChunks = 16;         // chunks per big-row window
+constexpr int kBigMaxWords = kBigMaxChunks * kChunkWords;
 constexpr int kDenseCols = 16384;         // single dense window (64 KB acc)
 constexpr int kHashMaxNnz = 2048;         // numeric hash rows: nnz(C_i) <= this
 constexpr int kSymHashMaxU = 2048;        // symbolic hash rows: min(ops,span) <= this
@@ -130,6 +134,31 @@ __device__ __forceinline__ int group_sum(int v)
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
     return v;
+}
+
+// Exclusive scan over a 256-thread block (4 waves); *total gets the sum.
+// Every thread of the block must call it.
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t *total)
+{
+    __shared__ int64_t wsum[4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int64_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int64_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    int64_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        if (w < wid) before += wsum[w];
+        all += wsum[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
 }
 
 // First index p in [lo, hi) of a sorted device array with a[p] >= key,

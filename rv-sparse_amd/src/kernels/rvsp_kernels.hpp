@@ -12,12 +12,29 @@ struct AnalysisCounters {
     int sym_hist[16];
     int num_hist[32];
     unsigned long long products;
-    unsigned long long big_cursor_cap;  // sum over big rows of alen*(nwin+1)
-    unsigned long long big_windows;     // sum over big rows of nwin
-    unsigned long long cursor_fill;     // running allocation of cursors
-    int nitems;                         // window items emitted
+    unsigned long long bits_cap;   // sum over big rows of bitmap words
+    unsigned long long bits_fill;  // running allocation of bitmap words
+    unsigned long long cur_total;  // cursor ints over all big-row segments
+    int nseg;                      // big-row segments
     int pad;
-    long long nnz_total;                // scan total
+    long long nnz_total;           // scan total
+};
+
+// Big-row state: bitmaps, chunk counts, windows, segments, cursors.
+struct BigArgs {
+    const int32_t *arp, *aci, *brp, *bci;
+    const int32_t *lo, *hi;
+    int32_t *cnt;
+    int64_t *big_off;     // per row: offset of its bitmap words in gbits
+    uint32_t *gbits;
+    int32_t *gchunk;      // popcount per chunk (index big_off/32 + chunk)
+    unsigned long long *bits_fill;
+    int32_t *win_chunk0;  // first chunk of each window (same indexing)
+    int32_t *win_out;     // first output (row-relative) of each window
+    int32_t *big_nwin;    // per row: number of windows
+    int4 *segs;           // (row, first window, end window, 0)
+    int64_t *seg_cur;     // per segment: offset of its cursor block
+    int32_t *cursors;
 };
 
 struct SymArgs {
@@ -26,9 +43,7 @@ struct SymArgs {
     int32_t *cnt;
     const int32_t *crp;
     int32_t *cci;
-    int64_t *cur_off;
-    int32_t *cursors;
-    unsigned long long *cursor_fill;
+    BigArgs big;
     int *err;
 };
 
@@ -40,8 +55,6 @@ struct NumArgs {
     const int32_t *crp, *cci;
     float *cval;
     const int32_t *lo;
-    const int64_t *cur_off;
-    const int32_t *cursors;
     int *err;
 };
 
@@ -58,9 +71,15 @@ void launch_scan(hipStream_t st, int32_t m, const int32_t *cnt, int64_t *tsum, i
                  int32_t *crp);
 int scan_tiles(int32_t m);
 void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_t *ops,
-                     const int32_t *arp, const uint8_t *sbin, const int32_t *lo,
-                     const int32_t *hi, uint8_t *nbin, int2 *items, AnalysisCounters *ctr);
+                     const int32_t *arp, const uint8_t *sbin, uint8_t *nbin,
+                     AnalysisCounters *ctr);
 void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
-                    const int32_t *off, const int2 *items, int nitems);
+                    const int32_t *off, const BigArgs &big, int nsegs);
+void launch_sym_big(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g);
+void launch_big_fill(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,
+                     const int32_t *crp, int32_t *cci);
+void launch_big_plan(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,
+                     const int32_t *crp, AnalysisCounters *ctr);
+void launch_num_big(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g);
 
 } // namespace rvsp

@@ -17,9 +17,7 @@
 //   hash    nnz(C_i) <= 2048: LDS table of H >= 2 nnz slots, pre-loaded with
 //           C_i's columns, G lanes per row (G ~ mean |B_k|)
 //   dense   one dense LDS window over the row's span (<= 16384 columns)
-//   window  rows the symbolic pass found big: kWinCols-column windows, each
-//           a dense LDS accumulator owned by one wave, with per-(A entry,
-//           window) cursors into B_k written by the symbolic pass
+//   big     rows the symbolic pass kept a bitmap for (rvsp_bigrow.hip)
 #include "rvsp_device.hpp"
 #include "rvsp_kernels.hpp"
 
@@ -78,79 +76,52 @@ __global__ __launch_bounds__(R *G) void k_num_hash(const int32_t *__restrict__ r
     }
 }
 
-// One wave per window. CURSORS: items are (row, window) pairs and B_k's
-// slice for the window comes from the symbolic pass's cursors. Otherwise
-// items are rows whose whole span fits one window.
-template <int WCOLS, int NW, bool CURSORS>
-__global__ __launch_bounds__(NW * 64) void k_num_window(const int32_t *__restrict__ rows,
-                                                        const int2 *__restrict__ items,
-                                                        int32_t nitems, NumArgs a)
+// One wave per row whose product columns span at most WCOLS columns: a
+// dense LDS accumulator over [lo, lo + WCOLS).
+template <int WCOLS>
+__global__ __launch_bounds__(64) void k_num_dense(const int32_t *__restrict__ rows, int32_t nrows,
+                                                  NumArgs a)
 {
-    __shared__ float acc[NW * WCOLS];
-    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int64_t it = (int64_t)blockIdx.x * NW + wv;
-    const bool valid = it < nitems;
-    int row = 0, win = 0;
-    if (valid) {
-        if constexpr (CURSORS) {
-            const int2 item = items[it];
-            row = item.x;
-            win = item.y;
-        } else {
-            row = rows[it];
-        }
-    }
-    float *S = acc + wv * WCOLS;
+    __shared__ float S[WCOLS];
+    const int lane = threadIdx.x;
+    const int row = rows[blockIdx.x];
     for (int i = lane; i < WCOLS; i += 64) S[i] = 0.0f;
-    const int32_t wlo = valid ? a.lo[row] + win * WCOLS : 0;
-    const int64_t whi64 = (int64_t)wlo + WCOLS;
-    const int32_t whi = whi64 > INT32_MAX ? INT32_MAX : (int32_t)whi64;
-    int p0 = 0, p1 = 0;
-    if (valid) {
-        const int cs = a.crp[row], ce = a.crp[row + 1];
-        p0 = wave_lower_bound(a.cci, cs, ce, wlo);
-        p1 = wave_lower_bound(a.cci, p0, ce, whi);
-    }
+    const int32_t wlo = a.lo[row];
     wave_sync();
-    if (valid && p1 > p0) {
-        const int as = a.arp[row], ae = a.arp[row + 1];
-        const int alen = ae - as;
-        const int64_t cbase = CURSORS ? a.cur_off[row] : 0;
-        for (int base = as; base < ae; base += 64) {
-            const int ap = base + lane;
-            float av = 0.0f;
-            int b0 = 0, b1 = 0;
-            if (ap < ae) {
-                av = a.aval[ap];
-                if constexpr (CURSORS) {
-                    b0 = a.cursors[cbase + (int64_t)win * alen + (ap - as)];
-                    b1 = a.cursors[cbase + (int64_t)(win + 1) * alen + (ap - as)];
-                } else {
-                    const int k = a.aci[ap];
-                    b0 = a.brp[k];
-                    b1 = a.brp[k + 1];
-                }
+    const int as = a.arp[row], ae = a.arp[row + 1];
+    for (int base = as; base < ae; base += 64) {
+        const int ap = base + lane;
+        float av = 0.0f;
+        int b0 = 0, b1 = 0;
+        if (ap < ae) {
+            av = a.aval[ap];
+            const int k = a.aci[ap];
+            b0 = a.brp[k];
+            b1 = a.brp[k + 1];
+        }
+        uint64_t live = __ballot(b1 > b0);
+        while (live) {
+            const int j = __ffsll((unsigned long long)live) - 1;
+            live &= live - 1;
+            const float aj = __shfl(av, j, 64);
+            const int sj = __shfl(b0, j, 64), ej = __shfl(b1, j, 64);
+            for (int bp = sj + lane; bp < ej; bp += 64) {
+                const int32_t c = a.bci[bp] - wlo;
+                const float prod = __fmul_rn(aj, a.bval[bp]);
+                if ((uint32_t)c < (uint32_t)WCOLS)
+                    atomicAdd(&S[c], prod);
+                else
+                    atomicOr(a.err, kErrWindow);
             }
-            uint64_t live = __ballot(b1 > b0);
-            while (live) {
-                const int j = __ffsll((unsigned long long)live) - 1;
-                live &= live - 1;
-                const float aj = __shfl(av, j, 64);
-                const int sj = __shfl(b0, j, 64), ej = __shfl(b1, j, 64);
-                for (int bp = sj + lane; bp < ej; bp += 64) {
-                    const int32_t c = a.bci[bp] - wlo;
-                    const float prod = __fmul_rn(aj, a.bval[bp]);
-                    if ((uint32_t)c < (uint32_t)WCOLS)
-                        atomicAdd(&S[c], prod);
-                    else
-                        atomicOr(a.err, kErrWindow);
-                }
-                compiler_fence();
-            }
+            compiler_fence();
         }
     }
     wave_sync();
-    for (int q = p0 + lane; q < p1; q += 64) a.cval[q] = S[a.cci[q] - wlo];
+    const int cs = a.crp[row], ce = a.crp[row + 1];
+    for (int q = cs + lane; q < ce; q += 64) {
+        const int32_t c = a.cci[q] - wlo;
+        a.cval[q] = (uint32_t)c < (uint32_t)WCOLS ? S[c] : 0.0f;
+    }
 }
 
 template <int G, int H>
@@ -168,7 +139,7 @@ static void launch_hash(hipStream_t st, const NumArgs &a, const int32_t *rows, i
         break;
 
 void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
-                    const int32_t *off, const int2 *items, int nitems)
+                    const int32_t *off, const BigArgs &big, int nsegs)
 {
     for (int b = NB_HASH0; b < NB_COUNT; ++b) {
         const int nr = count[b];
@@ -201,17 +172,13 @@ void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const
             RVSP_HASH_CASE(6, 2, 4096, 32)
             RVSP_HASH_CASE(6, 3, 4096, 64)
         case NB_DENSE:
-            k_num_window<kDenseCols, 1, false><<<nr, 64, 0, st>>>(rows, nullptr, nr, a);
+            k_num_dense<kDenseCols><<<nr, 64, 0, st>>>(rows, nr, a);
             break;
         default:
             break;
         }
     }
-    if (nitems > 0) {
-        constexpr int NW = 4;
-        k_num_window<kWinCols, NW, true><<<(nitems + NW - 1) / NW, NW * 64, 0, st>>>(
-            nullptr, items, nitems, a);
-    }
+    launch_num_big(st, nsegs, a, big);
 }
 
 } // namespace rvsp

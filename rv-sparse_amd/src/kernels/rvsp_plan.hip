@@ -136,10 +136,10 @@ struct rvsp_hip_plan {
     int32_t *binoff = nullptr;  // [0,32) offsets, [32,64) cursors
     int *err = nullptr;
 
-    int64_t *cur_off = nullptr;
+    void *bigblock = nullptr;  // big-row bitmaps, chunk counts, windows, segments
     int32_t *cursors = nullptr;
-    int2 *items = nullptr;
-    int nitems = 0;
+    BigArgs big{};
+    int nseg = 0;
 
     AnalysisCounters hctr;
     int32_t sym_count[SB_COUNT], sym_off[SB_COUNT];
@@ -155,14 +155,13 @@ struct rvsp_hip_plan {
 static void plan_release(rvsp_hip_plan *p)
 {
     dfree(p->rowblock, p->stream);
-    dfree(p->cur_off, p->stream);
+    dfree(p->bigblock, p->stream);
     dfree(p->cursors, p->stream);
-    dfree(p->items, p->stream);
     p->rowblock = nullptr;
-    p->cur_off = nullptr;
+    p->bigblock = nullptr;
     p->cursors = nullptr;
-    p->items = nullptr;
-    p->nitems = 0;
+    p->big = BigArgs{};
+    p->nseg = 0;
     p->analysed = false;
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
@@ -289,20 +288,46 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     launch_bin_scatter(st, m, p->sbin, p->binoff, p->binoff + 32, p->sperm, SB_COUNT);
     HIP_TRY(hipGetLastError());
 
-    if (p->sym_count[SB_BIG] > 0) {
-        s = dalloc((void **)&p->cur_off, M * 8, st);
-        if (s == RVSP_SUCCESS)
-            s = dalloc((void **)&p->cursors, (size_t)p->hctr.big_cursor_cap * 4, st);
-        if (s == RVSP_SUCCESS) s = dalloc((void **)&p->items, (size_t)p->hctr.big_windows * 8, st);
+    const int nbig = p->sym_count[SB_BIG];
+    p->big = BigArgs{};
+    p->big.arp = arp;
+    p->big.aci = aci;
+    p->big.brp = brp;
+    p->big.bci = bci;
+    p->big.lo = p->lo;
+    p->big.hi = p->hi;
+    p->big.cnt = p->cnt;
+    p->big.bits_fill = &p->ctr->bits_fill;
+    if (nbig > 0) {
+        const size_t words = (size_t)p->hctr.bits_cap;
+        const size_t chunks = words / kChunkWords;
+        size_t bo = 0;
+        const size_t o_off = bo; bo = align256(bo + M * 8);
+        const size_t o_nwin = bo; bo = align256(bo + M * 4);
+        const size_t o_bits = bo; bo = align256(bo + words * 4);
+        const size_t o_chunk = bo; bo = align256(bo + chunks * 4);
+        const size_t o_wc0 = bo; bo = align256(bo + chunks * 4);
+        const size_t o_wout = bo; bo = align256(bo + chunks * 4);
+        const size_t o_segs = bo; bo = align256(bo + chunks * 16);
+        const size_t o_scur = bo; bo = align256(bo + chunks * 8);
+        s = dalloc(&p->bigblock, bo, st);
         if (s != RVSP_SUCCESS) {
             plan_release(p);
             return s;
         }
+        char *bb = (char *)p->bigblock;
+        p->big.big_off = (int64_t *)(bb + o_off);
+        p->big.big_nwin = (int32_t *)(bb + o_nwin);
+        p->big.gbits = (uint32_t *)(bb + o_bits);
+        p->big.gchunk = (int32_t *)(bb + o_chunk);
+        p->big.win_chunk0 = (int32_t *)(bb + o_wc0);
+        p->big.win_out = (int32_t *)(bb + o_wout);
+        p->big.segs = (int4 *)(bb + o_segs);
+        p->big.seg_cur = (int64_t *)(bb + o_scur);
     }
 
     // ---- 2. exact nnz(C_i)
-    SymArgs sa{arp, aci, brp, bci, p->lo, p->hi, p->cnt, p->crp, nullptr,
-               p->cur_off, p->cursors, &p->ctr->cursor_fill, p->err};
+    SymArgs sa{arp, aci, brp, bci, p->lo, p->hi, p->cnt, p->crp, nullptr, p->big, p->err};
     // rows without products get no kernel, so their count must start at 0
     HIP_TRY(hipMemsetAsync(p->cnt, 0, M * 4, st));
     launch_sym_count(st, sa, p->sperm, p->sym_count, p->sym_off);
@@ -310,13 +335,25 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     // ---- 3. row pointer
     launch_scan(st, m, p->cnt, p->tsum, (int64_t *)&p->ctr->nnz_total, p->crp);
     HIP_TRY(hipGetLastError());
-    // ---- 4. numeric bins
-    launch_num_bins(st, m, p->crp, p->ops, arp, p->sbin, p->lo, p->hi, p->nbin, p->items, p->ctr);
+    // ---- 4. big-row windows and segments, numeric bins
+    if (nbig > 0) {
+        launch_big_plan(st, p->sperm + p->sym_off[SB_BIG], nbig, p->big, p->crp, p->ctr);
+        HIP_TRY(hipGetLastError());
+    }
+    launch_num_bins(st, m, p->crp, p->ops, arp, p->sbin, p->nbin, p->ctr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&p->hctr, p->ctr, sizeof(AnalysisCounters), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     p->nnzc = p->hctr.nnz_total;
-    p->nitems = p->hctr.nitems;
+    p->nseg = p->hctr.nseg;
+    if (p->nseg > 0) {
+        s = dalloc((void **)&p->cursors, (size_t)p->hctr.cur_total * 4, st);
+        if (s != RVSP_SUCCESS) {
+            plan_release(p);
+            return s;
+        }
+        p->big.cursors = p->cursors;
+    }
     {
         int run = 0;
         memset(hoff, 0, sizeof(hoff));
@@ -345,8 +382,7 @@ rvsp_status_t rvsp_hip_plan_fill(rvsp_hip_plan_t *p, int32_t *cci)
 {
     if (!p || !cci) return RVSP_ERROR_NULL_POINTER;
     if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
-    SymArgs sa{p->arp, p->aci, p->brp, p->bci, p->lo, p->hi, p->cnt, p->crp, cci,
-               p->cur_off, p->cursors, &p->ctr->cursor_fill, p->err};
+    SymArgs sa{p->arp, p->aci, p->brp, p->bci, p->lo, p->hi, p->cnt, p->crp, cci, p->big, p->err};
     PhaseScope timer(p->stream, PH_FILL);
     launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off);
     HIP_TRY(hipGetLastError());
@@ -356,8 +392,7 @@ rvsp_status_t rvsp_hip_plan_fill(rvsp_hip_plan_t *p, int32_t *cci)
 static NumArgs num_args(rvsp_hip_plan *p, const float *aval, const float *bval, const int32_t *cci,
                         float *cval)
 {
-    return NumArgs{p->arp, p->aci, aval,    p->brp, p->bci,    bval,    p->crp,
-                   cci,    cval,   p->lo,   p->cur_off, p->cursors, p->err};
+    return NumArgs{p->arp, p->aci, aval, p->brp, p->bci, bval, p->crp, cci, cval, p->lo, p->err};
 }
 
 rvsp_status_t rvsp_hip_plan_numeric(rvsp_hip_plan_t *p, const float *aval, const float *bval,
@@ -367,7 +402,7 @@ rvsp_status_t rvsp_hip_plan_numeric(rvsp_hip_plan_t *p, const float *aval, const
     if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
     PhaseScope timer(p->stream, PH_NUMERIC);
     launch_numeric(p->stream, num_args(p, aval, bval, cci, cval), p->nperm, p->num_count,
-                   p->num_off, p->items, p->nitems);
+                   p->num_off, p->big, p->nseg);
     HIP_TRY(hipGetLastError());
     return RVSP_SUCCESS;
 }
@@ -387,7 +422,7 @@ rvsp_status_t rvsp_hip_plan_numeric_graph(rvsp_hip_plan_t *p, const float *aval,
     if (!p->gexec) {
         HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
         launch_numeric(p->stream, num_args(p, aval, bval, cci, cval), p->nperm, p->num_count,
-                       p->num_off, p->items, p->nitems);
+                       p->num_off, p->big, p->nseg);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(p->stream, &g);
         if (e != hipSuccess) return map_err(e);

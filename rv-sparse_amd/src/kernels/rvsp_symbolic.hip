@@ -74,11 +74,8 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
         sbin_out[row] = (uint8_t)b;
         atomicAdd(&hist[b], 1);
         if (b == SB_BIG) {
-            const int alen = arp[row + 1] - arp[row];
-            const int nwin = (int)(((int64_t)hi - lo + kWinCols) / kWinCols);
-            atomicAdd((unsigned long long *)&ctr->big_cursor_cap,
-                      (unsigned long long)alen * (unsigned long long)(nwin + 1));
-            atomicAdd((unsigned long long *)&ctr->big_windows, (unsigned long long)nwin);
+            const int nchunks = (int)(((int64_t)hi - lo + kChunkCols) / kChunkCols);
+            atomicAdd(&ctr->bits_cap, (unsigned long long)nchunks * kChunkWords);
         }
         atomicAdd((unsigned long long *)&ctr->products, (unsigned long long)ops);
     }
@@ -196,28 +193,22 @@ __global__ __launch_bounds__(256) void k_sym_hash(const int32_t *__restrict__ ro
 }
 
 // ---------------------------------------------------------------------------
-// Symbolic pass, bitmap variant: one bit per column of a window of
-// 32*WORDS columns starting at the row's smallest product column. Small
-// rows (span <= 16384) take one pass; big rows loop windows of 2^19
-// columns. Fill mode emits set bits in ascending order, which is C_i's
-// column order by construction. In count mode big rows also record, for
-// every A entry and every numeric window of kWinCols columns, where that
-// window starts inside B_k (the numeric window kernel's cursors).
+// Symbolic pass, bitmap variant for rows whose product columns span at most
+// 32*WORDS columns: one bit per column from the row's smallest product
+// column. Fill mode emits set bits in ascending order, which is C_i's column
+// order by construction. (Wider rows go to the big-row path,
+// rvsp_bigrow.hip.)
 // ---------------------------------------------------------------------------
-template <int G, int WORDS, bool FILL, bool BIG>
+template <int G, int WORDS, bool FILL>
 __global__ __launch_bounds__(256) void k_sym_bitmap(const int32_t *__restrict__ rows, int32_t nrows,
                                                     const int32_t *__restrict__ arp,
                                                     const int32_t *__restrict__ aci,
                                                     const int32_t *__restrict__ brp,
                                                     const int32_t *__restrict__ bci,
                                                     const int32_t *__restrict__ lo_col,
-                                                    const int32_t *__restrict__ hi_col,
                                                     int32_t *__restrict__ cnt,
                                                     const int32_t *__restrict__ crp,
-                                                    int32_t *__restrict__ cci,
-                                                    int64_t *__restrict__ cur_off,
-                                                    int32_t *__restrict__ cursors,
-                                                    unsigned long long *cursor_fill, int *err)
+                                                    int32_t *__restrict__ cci)
 {
     constexpr int R = 256 / G;
     constexpr int COLS = 32 * WORDS;
@@ -225,102 +216,55 @@ __global__ __launch_bounds__(256) void k_sym_bitmap(const int32_t *__restrict__ 
     static_assert(WORDS % G == 0, "words per thread");
     __shared__ uint32_t bits[R * WORDS];
     __shared__ int scan[256];
-    __shared__ long long cbase_s[R];
     const int g = threadIdx.x / G, lane = threadIdx.x % G;
     const int64_t ridx = (int64_t)blockIdx.x * R + g;
     const bool valid = ridx < nrows;
     const int row = valid ? rows[ridx] : 0;
     uint32_t *Bm = bits + g * WORDS;
-    const int32_t lo = valid ? lo_col[row] : 0;
-    const int32_t hi = valid ? hi_col[row] : -1;
-    const int as = valid ? arp[row] : 0, ae = valid ? arp[row + 1] : 0;
-    const int alen = ae - as;
-    int64_t cbase = 0;
-    int nwin = 0;
-    if constexpr (BIG && !FILL) {
-        nwin = (int)(((int64_t)hi - lo + kWinCols) / kWinCols);
-        if (valid && lane == 0) {
-            cbase = (long long)atomicAdd(cursor_fill, (unsigned long long)alen * (unsigned long long)(nwin + 1));
-            cur_off[row] = cbase;
-            cbase_s[g] = cbase;
-        }
-        __syncthreads();
-        cbase = cbase_s[g];
-    }
-    // BIG rows use one block per row (R == 1), so the pass count is uniform
-    // across the block; small rows always take a single pass.
-    const int npass = BIG ? (int)(((int64_t)hi - lo + COLS) / COLS) : 1;
-    int total = 0;
-    int out = (FILL && valid) ? crp[row] : 0;
-    for (int pass = 0; pass < npass; ++pass) {
-        const int32_t base = lo + pass * COLS;
-        for (int w = lane; w < WORDS; w += G) Bm[w] = 0u;
-        __syncthreads();
-        if (valid) {
-            for (int ap = as; ap < ae; ++ap) {
-                const int k = aci[ap];
-                const int bs = brp[k], be = brp[k + 1];
-                for (int bp = bs + lane; bp < be; bp += G) {
-                    const int32_t col = bci[bp];
-                    const int32_t c = col - base;
-                    if (c >= 0 && c < COLS) atomicOr(&Bm[c >> 5], 1u << (c & 31));
-                    if constexpr (BIG && !FILL) {
-                        if (pass == 0) {
-                            // window boundaries crossed between the previous
-                            // entry and this one start at bp
-                            const int wcur = (col - lo) / kWinCols;
-                            const int wprev = bp > bs ? (bci[bp - 1] - lo) / kWinCols : -1;
-                            for (int w = wprev + 1; w <= wcur; ++w)
-                                cursors[cbase + (int64_t)w * alen + (ap - as)] = bp;
-                            if (bp == be - 1)
-                                for (int w = wcur + 1; w <= nwin; ++w)
-                                    cursors[cbase + (int64_t)w * alen + (ap - as)] = be;
-                        }
-                    }
-                }
-                if constexpr (BIG && !FILL) {
-                    if (pass == 0 && be == bs && lane == 0)
-                        for (int w = 0; w <= nwin; ++w) cursors[cbase + (int64_t)w * alen + (ap - as)] = bs;
-                }
+    const int32_t base = valid ? lo_col[row] : 0;
+    for (int w = lane; w < WORDS; w += G) Bm[w] = 0u;
+    __syncthreads();
+    if (valid) {
+        const int as = arp[row], ae = arp[row + 1];
+        for (int ap = as; ap < ae; ++ap) {
+            const int k = aci[ap];
+            const int bs = brp[k], be = brp[k + 1];
+            for (int bp = bs + lane; bp < be; bp += G) {
+                const int32_t c = bci[bp] - base;
+                if (c >= 0 && c < COLS) atomicOr(&Bm[c >> 5], 1u << (c & 31));
             }
         }
-        __syncthreads();
-        // ordered scan: thread `lane` owns words [lane*WPT, (lane+1)*WPT)
-        int local = 0;
+    }
+    __syncthreads();
+    // ordered scan: thread `lane` owns words [lane*WPT, (lane+1)*WPT)
+    int local = 0;
 #pragma unroll 4
-        for (int w = 0; w < WPT; ++w) local += __popc(Bm[lane * WPT + w]);
-        scan[threadIdx.x] = local;
+    for (int w = 0; w < WPT; ++w) local += __popc(Bm[lane * WPT + w]);
+    scan[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < G; off <<= 1) {
+        const int v = lane >= off ? scan[threadIdx.x - off] : 0;
         __syncthreads();
-        // exclusive prefix over the group (G <= 256, simple serial-in-LDS
-        // Hillis-Steele)
-        for (int off = 1; off < G; off <<= 1) {
-            const int v = lane >= off ? scan[threadIdx.x - off] : 0;
-            __syncthreads();
-            scan[threadIdx.x] += v;
-            __syncthreads();
-        }
-        const int incl = scan[threadIdx.x];
-        const int gtotal = scan[g * G + G - 1];
-        if constexpr (FILL) {
-            if (valid) {
-                int pos = out + incl - local;
-                for (int w = 0; w < WPT; ++w) {
-                    uint32_t word = Bm[lane * WPT + w];
-                    const int32_t cb = base + (lane * WPT + w) * 32;
-                    while (word) {
-                        const int b = __ffs(word) - 1;
-                        word &= word - 1;
-                        cci[pos++] = cb + b;
-                    }
-                }
-            }
-            out += gtotal;
-        }
-        total += gtotal;
+        scan[threadIdx.x] += v;
         __syncthreads();
     }
-    if constexpr (!FILL) {
-        if (valid && lane == 0) cnt[row] = total;
+    const int incl = scan[threadIdx.x];
+    const int gtotal = scan[g * G + G - 1];
+    if constexpr (FILL) {
+        if (valid) {
+            int pos = crp[row] + incl - local;
+            for (int w = 0; w < WPT; ++w) {
+                uint32_t word = Bm[lane * WPT + w];
+                const int32_t cb = base + (lane * WPT + w) * 32;
+                while (word) {
+                    const int b = __ffs(word) - 1;
+                    word &= word - 1;
+                    cci[pos++] = cb + b;
+                }
+            }
+        }
+    } else {
+        if (valid && lane == 0) cnt[row] = gtotal;
     }
 }
 
@@ -331,29 +275,6 @@ __global__ __launch_bounds__(256) void k_sym_bitmap(const int32_t *__restrict__ 
 // ---------------------------------------------------------------------------
 constexpr int kScanItems = 8;
 constexpr int kScanTile = 256 * kScanItems;
-
-__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t *total)
-{
-    __shared__ int64_t wsum[4];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int64_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int64_t y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) wsum[wid] = x;
-    __syncthreads();
-    int64_t before = 0, all = 0;
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-        if (w < wid) before += wsum[w];
-        all += wsum[w];
-    }
-    __syncthreads();
-    *total = all;
-    return before + x - v;
-}
 
 __global__ __launch_bounds__(256) void k_scan_tiles(int32_t m, const int32_t *__restrict__ cnt,
                                                     int64_t *__restrict__ tsum)
@@ -409,8 +330,8 @@ __global__ __launch_bounds__(256) void k_scan_apply(int32_t m, const int32_t *__
 
 // ---------------------------------------------------------------------------
 // Numeric binning from exact nnz(C_i): LDS hash (H from nnz, G from the
-// mean B-row length), a single dense window, or column windows (rows the
-// symbolic pass treated as big, which have cursors).
+// mean B-row length), a single dense window, or the big-row path (rows the
+// symbolic pass kept a bitmap for).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int num_hash_bin(int nz, int64_t ops, int alen)
 {
@@ -428,10 +349,7 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
                                                   const int64_t *__restrict__ ops,
                                                   const int32_t *__restrict__ arp,
                                                   const uint8_t *__restrict__ sbin,
-                                                  const int32_t *__restrict__ lo_col,
-                                                  const int32_t *__restrict__ hi_col,
                                                   uint8_t *__restrict__ nbin,
-                                                  int2 *__restrict__ items,
                                                   AnalysisCounters *__restrict__ ctr)
 {
     __shared__ int hist[NB_COUNT];
@@ -441,18 +359,14 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
     if (i < m) {
         const int nz = crp[i + 1] - crp[i];
         int b;
-        if (nz == 0) {
+        if (nz == 0)
             b = NB_EMPTY;
-        } else if (sbin[i] == SB_BIG) {
-            b = NB_WIN;
-            const int nwin = (int)(((int64_t)hi_col[i] - lo_col[i] + kWinCols) / kWinCols);
-            const int base = atomicAdd(&ctr->nitems, nwin);
-            for (int w = 0; w < nwin; ++w) items[base + w] = make_int2((int)i, w);
-        } else if (nz <= kHashMaxNnz) {
+        else if (sbin[i] == SB_BIG)
+            b = NB_WIN;  // segments planned by k_big_plan
+        else if (nz <= kHashMaxNnz)
             b = num_hash_bin(nz, ops[i], arp[i + 1] - arp[i]);
-        } else {
+        else
             b = NB_DENSE;
-        }
         nbin[i] = (uint8_t)b;
         atomicAdd(&hist[b], 1);
     }
@@ -491,15 +405,15 @@ static void launch_sym_bins(hipStream_t st, const SymArgs &a, const int32_t *per
         case SB_H4096: launch_sym_hash<256, 4096, FILL>(st, a, rows, nr); break;
         case SB_BMS: {
             const int grid = (nr + 3) / 4;
-            k_sym_bitmap<64, kBmsWords, FILL, false><<<grid, 256, 0, st>>>(
-                rows, nr, a.arp, a.aci, a.brp, a.bci, a.lo, a.hi, a.cnt, a.crp, a.cci, a.cur_off,
-                a.cursors, a.cursor_fill, a.err);
+            k_sym_bitmap<64, kBmsWords, FILL><<<grid, 256, 0, st>>>(
+                rows, nr, a.arp, a.aci, a.brp, a.bci, a.lo, a.cnt, a.crp, a.cci);
             break;
         }
         case SB_BIG:
-            k_sym_bitmap<256, kBigWords, FILL, true><<<nr, 256, 0, st>>>(
-                rows, nr, a.arp, a.aci, a.brp, a.bci, a.lo, a.hi, a.cnt, a.crp, a.cci, a.cur_off,
-                a.cursors, a.cursor_fill, a.err);
+            if constexpr (FILL)
+                launch_big_fill(st, rows, nr, a.big, a.crp, a.cci);
+            else
+                launch_sym_big(st, rows, nr, a.big);
             break;
         default: break;
         }
@@ -550,11 +464,11 @@ int scan_tiles(int32_t m)
 }
 
 void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_t *ops,
-                     const int32_t *arp, const uint8_t *sbin, const int32_t *lo,
-                     const int32_t *hi, uint8_t *nbin, int2 *items, AnalysisCounters *ctr)
+                     const int32_t *arp, const uint8_t *sbin, uint8_t *nbin,
+                     AnalysisCounters *ctr)
 {
     const int grid = (int)(((int64_t)m + 255) / 256);
-    k_num_bins<<<grid, 256, 0, st>>>(m, crp, ops, arp, sbin, lo, hi, nbin, items, ctr);
+    k_num_bins<<<grid, 256, 0, st>>>(m, crp, ops, arp, sbin, nbin, ctr);
 }
 
 } // namespace rvsp
