@@ -58,27 +58,32 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
         for (int ap = as; ap < ae; ++ap) {
             const int k = g.aci[ap];
             const int bs = g.brp[k], be = g.brp[k + 1];
-            int bp = bs + tid;
-            int32_t nxt = bp < be ? g.bci[bp] : -1;
-            for (int base = bs; base < be; base += NT) {
-                const int32_t col = nxt;
-                const bool valid = bp < be;
-                const int bn = bp + NT;
-                nxt = bn < be ? g.bci[bn] : -1;  // next load in flight
-                const int32_t c = col - base_col;
-                const bool in = valid && c >= 0 && c < pcols;
-                int wd = in ? (c >> 5) : -1 - lane;
-                uint32_t wb = in ? (1u << (c & 31)) : 0u;
+            // 4 entries per thread in flight, then OR-combine each slot
+            for (int base = bs; base < be; base += 4 * NT) {
+                int32_t col[4];
 #pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    const uint32_t ob = __shfl_up(wb, o, 64);
-                    const int ow = __shfl_up(wd, o, 64);
-                    if (lane >= o && ow == wd) wb |= ob;
+                for (int u = 0; u < 4; ++u) {
+                    const int bp = base + u * NT + tid;
+                    col[u] = bp < be ? g.bci[bp] : -1;
                 }
-                const int nwd = __shfl_down(wd, 1, 64);
-                const bool last = lane == 63 || nwd != wd;
-                if (in && last) atomicOr(&bits[wd], wb);
-                bp = bn;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (base + u * NT >= be) break;  // block-uniform
+                    const bool valid = base + u * NT + tid < be;
+                    const int32_t c = col[u] - base_col;
+                    const bool in = valid && c >= 0 && c < pcols;
+                    int wd = in ? (c >> 5) : -1 - lane;
+                    uint32_t wb = in ? (1u << (c & 31)) : 0u;
+#pragma unroll
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const uint32_t ob = __shfl_up(wb, o, 64);
+                        const int ow = __shfl_up(wd, o, 64);
+                        if (lane >= o && ow == wd) wb |= ob;
+                    }
+                    const int nwd = __shfl_down(wd, 1, 64);
+                    const bool last = lane == 63 || nwd != wd;
+                    if (in && last) atomicOr(&bits[wd], wb);
+                }
             }
         }
         __syncthreads();
@@ -225,22 +230,55 @@ __device__ __forceinline__ int lane_lower_bound(const int32_t *__restrict__ a, i
     return lo;
 }
 
+// Adds the products of one A entry's piece in [c0, c1) that start at `cj`
+// (loaded values already in col/val for the first 64 entries); returns the
+// new cursor. Lanes cover consecutive entries of B_k, so columns are
+// distinct within an instruction.
+__device__ __forceinline__ int big_piece(int cj, int ej, float aj, int32_t col, float bv,
+                                         int32_t c0, int32_t c1, int outs, const uint32_t *W,
+                                         const uint16_t *P, float *acc, const NumArgs &a)
+{
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        const bool in = col < c1;
+        if (in) {
+            const int rel = col - c0;
+            if (rel >= 0) {
+                const int wi = rel >> 5;
+                const uint32_t word = W[wi];
+                const int pos = (int)P[wi] + __popc(word & ((1u << (rel & 31)) - 1u));
+                if (((word >> (rel & 31)) & 1u) && pos < outs)
+                    atomicAdd(&acc[pos], __fmul_rn(aj, bv));
+                else
+                    atomicOr(a.err, kErrProbeMiss);
+            } else {
+                atomicOr(a.err, kErrWindow);
+            }
+        }
+        const int nin = __popcll(__ballot(in));
+        cj += nin;
+        if (nin < 64) return cj;
+        const int bp = cj + lane;  // piece continues past 64 entries
+        col = bp < ej ? a.bci[bp] : INT32_MAX;
+        bv = bp < ej ? a.bval[bp] : 0.0f;
+    }
+}
+
 template <int NW>
 __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, BigArgs g)
 {
     __shared__ float acc_s[NW][kBigPW];
     __shared__ uint32_t W_s[NW][kBigMaxWords];
-    __shared__ int P_s[NW][kBigMaxWords];
+    __shared__ uint16_t P_s[NW][kBigMaxWords];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t it = (int64_t)blockIdx.x * NW + wv;
     if (it >= nsegs) return;  // whole wave leaves; no block barriers below
     float *acc = acc_s[wv];
     uint32_t *W = W_s[wv];
-    int *P = P_s[wv];
+    uint16_t *P = P_s[wv];
     const int4 sg = g.segs[it];
     const int row = sg.x, w0 = sg.y, w1 = sg.z;
-    const int64_t cb = g.seg_cur[it];
-    int32_t *cur_a = g.cursors + cb;
+    int2 *cur_a = g.cursors + g.seg_cur[it];
     const int as = a.arp[row], ae = a.arp[row + 1];
     const int32_t rlo = g.lo[row], rhi = g.hi[row];
     const int nchunks = (int)(((int64_t)rhi - rlo + kChunkCols) / kChunkCols);
@@ -250,7 +288,7 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
     const int rowbase = a.crp[row];
     const int nz = a.crp[row + 1] - rowbase;
 
-    // cursors at the segment's first column
+    // per-entry state at the segment's first column: (cursor, end of B_k)
     const int32_t cstart = rlo + g.win_chunk0[c32 + w0] * kChunkCols;
     for (int base = as; base < ae; base += 64) {
         const int ap = base + lane;
@@ -259,7 +297,7 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
             int s = a.brp[k];
             const int e = a.brp[k + 1];
             if (w0 > 0) s = lane_lower_bound(a.bci, s, e, cstart);
-            cur_a[ap - as] = s;
+            cur_a[ap - as] = make_int2(s, e);
         }
     }
 
@@ -294,7 +332,7 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
         for (int j = 0; j < PER; ++j) {
             const int i = lane * PER + j;
             W[i] = wl[j];
-            P[i] = run;
+            P[i] = (uint16_t)run;
             run += __popc(wl[j]);
         }
         for (int i = lane; i < outs; i += 64) acc[i] = 0.0f;
@@ -302,46 +340,53 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
 
         for (int base = as; base < ae; base += 64) {
             const int ap = base + lane;
-            int cur = 0, e = 0;
+            int2 ce = make_int2(0, 0);
             float av = 0.0f;
             bool live = false;
             if (ap < ae) {
-                cur = cur_a[ap - as];
-                e = a.brp[a.aci[ap] + 1];
+                ce = cur_a[ap - as];
                 av = a.aval[ap];
-                live = cur < e && a.bci[cur] < c1;
+                live = ce.x < ce.y && a.bci[ce.x] < c1;
             }
+            int cur = ce.x;
             uint64_t mask = __ballot(live);
             while (mask) {
-                const int j = __ffsll((unsigned long long)mask) - 1;
-                mask &= mask - 1;
-                const float aj = __shfl(av, j, 64);
-                int cj = __shfl(cur, j, 64);
-                const int ej = __shfl(e, j, 64);
-                for (;;) {
-                    const int bp = cj + lane;
-                    const int32_t col = bp < ej ? a.bci[bp] : INT32_MAX;
-                    const bool in = col < c1;
-                    if (in && col < c0) atomicOr(a.err, kErrWindow);
-                    if (in && col >= c0) {
-                        const int rel = col - c0;
-                        const int wi = rel >> 5;
-                        const uint32_t word = W[wi];
-                        const int pos = P[wi] + __popc(word & ((1u << (rel & 31)) - 1u));
-                        const float prod = __fmul_rn(aj, a.bval[bp]);
-                        if (((word >> (rel & 31)) & 1u) && pos < outs)
-                            atomicAdd(&acc[pos], prod);
-                        else
-                            atomicOr(a.err, kErrProbeMiss);
+                // up to four live entries: issue all their loads first
+                int js[4], cs[4], es[4];
+                float as4[4], bvs[4];
+                int32_t cols[4];
+                int nj = 0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    js[u] = -1;
+                    if (mask) {
+                        js[u] = __ffsll((unsigned long long)mask) - 1;
+                        mask &= mask - 1;
+                        nj = u + 1;
                     }
-                    const int nin = __popcll(__ballot(in));
-                    cj += nin;
-                    if (nin < 64) break;
                 }
-                compiler_fence();
-                if (lane == j) cur = cj;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int ju = js[u] < 0 ? 0 : js[u];
+                    cs[u] = __shfl(ce.x, ju, 64);
+                    es[u] = __shfl(ce.y, ju, 64);
+                    as4[u] = __shfl(av, ju, 64);
+                    const int bp = cs[u] + lane;
+                    const bool ok = u < nj && bp < es[u];
+                    cols[u] = ok ? a.bci[bp] : INT32_MAX;
+                    bvs[u] = ok ? a.bval[bp] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (u < nj) {
+                        const int nc = big_piece(cs[u], es[u], as4[u], cols[u], bvs[u], c0, c1, outs,
+                                                 W, P, acc, a);
+                        compiler_fence();
+                        if (lane == js[u]) cur = nc;
+                    }
+                }
             }
-            if (ap < ae) cur_a[ap - as] = cur;
+            if (ap < ae) cur_a[ap - as].x = cur;
         }
         wave_sync();
         for (int i = lane; i < outs; i += 64) a.cval[rowbase + o0 + i] = acc[i];
@@ -371,7 +416,7 @@ void launch_big_plan(hipStream_t st, const int32_t *rows, int nrows, const BigAr
 
 void launch_num_big(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g)
 {
-    constexpr int NW = 4;
+    constexpr int NW = 2;
     if (nsegs > 0) k_num_big<NW><<<(nsegs + NW - 1) / NW, NW * 64, 0, st>>>(nsegs, a, g);
 }
 

@@ -40,8 +40,8 @@ constexpr int kBmsWords = 512;            // 16384 columns
 constexpr int kBigPassWords = 16384;      // big-row bitmap: 2^19 columns per pass
 constexpr int kChunkCols = 1024;          // big-row chunk (window granularity)
 constexpr int kChunkWords = kChunkCols / 32;
-constexpr int kBigPW = 2048;              // outputs per big-row window
-constexpr int kBigMaxChunks = 16;         // chunks per big-row window
+constexpr int kBigPW = 4096;              // outputs per big-row window
+constexpr int kBigMaxChunks = 32;         // chunks per big-row window
 constexpr int kBigMaxWords = kBigMaxChunks * kChunkWords;
 constexpr int kDenseCols = 16384;         // single dense window (64 KB acc)
 constexpr int kHashMaxNnz = 2048;         // numeric hash rows: nnz(C_i) <= this

@@ -34,7 +34,7 @@ struct BigArgs {
     int32_t *big_nwin;    // per row: number of windows
     int4 *segs;           // (row, first window, end window, 0)
     int64_t *seg_cur;     // per segment: offset of its cursor block
-    int32_t *cursors;
+    int2 *cursors;        // per (segment, A entry): (next B position, end of B row)
 };
 
 struct SymArgs {

@@ -137,7 +137,7 @@ struct rvsp_hip_plan {
     int *err = nullptr;
 
     void *bigblock = nullptr;  // big-row bitmaps, chunk counts, windows, segments
-    int32_t *cursors = nullptr;
+    int2 *cursors = nullptr;
     BigArgs big{};
     int nseg = 0;
 
@@ -347,7 +347,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->nnzc = p->hctr.nnz_total;
     p->nseg = p->hctr.nseg;
     if (p->nseg > 0) {
-        s = dalloc((void **)&p->cursors, (size_t)p->hctr.cur_total * 4, st);
+        s = dalloc((void **)&p->cursors, (size_t)p->hctr.cur_total * 8, st);
         if (s != RVSP_SUCCESS) {
             plan_release(p);
             return s;
