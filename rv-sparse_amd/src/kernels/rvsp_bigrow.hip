@@ -150,9 +150,10 @@ __global__ __launch_bounds__(256) void k_big_fill(const int32_t *__restrict__ ro
 // ---------------------------------------------------------------------------
 // Window and segment plan: one thread per big row. Windows are runs of
 // whole chunks with at most kBigPW outputs and kBigMaxChunks chunks.
-// Segments are runs of windows with about max(16384, 16*alen) outputs,
-// each processed by one wave; their cursor blocks (alen ints each) are
-// carved from a running counter.
+// Segments are runs of windows holding about kSegProducts products
+// (estimated from the row's products per output), each processed by one
+// wave, so the widest rows spread over many waves; their cursor blocks
+// (alen entries each) are carved from a running counter.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ rows, int32_t nrows,
                                                   BigArgs g, const int32_t *__restrict__ crp,
@@ -165,7 +166,9 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
     const int nchunks = (int)(((int64_t)rhi - rlo + kChunkCols) / kChunkCols);
     const int64_t c32 = g.big_off[row] / kChunkWords;
     const int alen = g.arp[row + 1] - g.arp[row];
-    const int seg_out = max(16384, 16 * alen);
+    const int nz = crp[row + 1] - crp[row];
+    // products per output of this row: segments aim at ~kSegProducts each
+    const double ratio = nz > 0 ? (double)g.ops[row] / (double)nz : 1.0;
     int nwin = 0, out = 0, acc = 0, start = 0;
     // pass 1: windows
     for (int c = 0; c < nchunks; ++c) {
@@ -184,33 +187,32 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
     g.win_out[c32 + nwin] = out;
     ++nwin;
     g.big_nwin[row] = nwin;
-    // pass 2: segments
+    // pass 2: segments (count, then emit)
     int nseg = 0;
-    {
-        int so = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        int sbase = 0;
+        long long cbase = 0;
+        if (pass == 1) {
+            sbase = atomicAdd(&ctr->nseg, nseg);
+            cbase = (long long)atomicAdd(&ctr->cur_total,
+                                         (unsigned long long)alen * (unsigned long long)nseg);
+        }
+        int s = 0, w0 = 0;
+        double est = 0.0;
         for (int w = 0; w < nwin; ++w) {
-            const int o1 = w + 1 < nwin ? g.win_out[c32 + w + 1] : crp[row + 1] - crp[row];
-            so += o1 - g.win_out[c32 + w];
-            if (so >= seg_out || w == nwin - 1) {
-                ++nseg;
-                so = 0;
+            const int o1 = w + 1 < nwin ? g.win_out[c32 + w + 1] : nz;
+            est += (o1 - g.win_out[c32 + w]) * ratio;
+            if (est >= (double)kSegProducts || w == nwin - 1) {
+                if (pass == 1) {
+                    g.segs[sbase + s] = make_int4(row, w0, w + 1, 0);
+                    g.seg_cur[sbase + s] = cbase + (long long)s * alen;
+                }
+                ++s;
+                w0 = w + 1;
+                est = 0.0;
             }
         }
-    }
-    const int sbase = atomicAdd(&ctr->nseg, nseg);
-    const long long cbase =
-        (long long)atomicAdd(&ctr->cur_total, (unsigned long long)alen * (unsigned long long)nseg);
-    int s = 0, w0 = 0, so = 0;
-    for (int w = 0; w < nwin; ++w) {
-        const int o1 = w + 1 < nwin ? g.win_out[c32 + w + 1] : crp[row + 1] - crp[row];
-        so += o1 - g.win_out[c32 + w];
-        if (so >= seg_out || w == nwin - 1) {
-            g.segs[sbase + s] = make_int4(row, w0, w + 1, 0);
-            g.seg_cur[sbase + s] = cbase + (long long)s * alen;
-            ++s;
-            w0 = w + 1;
-            so = 0;
-        }
+        nseg = s;
     }
 }
 
@@ -288,16 +290,41 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
     const int rowbase = a.crp[row];
     const int nz = a.crp[row + 1] - rowbase;
 
-    // per-entry state at the segment's first column: (cursor, end of B_k)
+    // per-entry state at the segment's first column: (cursor, end of B_k);
+    // four binary searches per lane run interleaved
     const int32_t cstart = rlo + g.win_chunk0[c32 + w0] * kChunkCols;
-    for (int base = as; base < ae; base += 64) {
-        const int ap = base + lane;
-        if (ap < ae) {
-            const int k = a.aci[ap];
-            int s = a.brp[k];
-            const int e = a.brp[k + 1];
-            if (w0 > 0) s = lane_lower_bound(a.bci, s, e, cstart);
-            cur_a[ap - as] = make_int2(s, e);
+    for (int base = as; base < ae; base += 4 * 64) {
+        int lo4[4], hi4[4], e4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int ap = base + u * 64 + lane;
+            lo4[u] = hi4[u] = e4[u] = 0;
+            if (ap < ae) {
+                const int k = a.aci[ap];
+                lo4[u] = a.brp[k];
+                e4[u] = a.brp[k + 1];
+                hi4[u] = w0 > 0 ? e4[u] : lo4[u];
+            }
+        }
+        for (;;) {
+            bool any = false;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (lo4[u] < hi4[u]) {
+                    any = true;
+                    const int mid = (lo4[u] + hi4[u]) >> 1;
+                    if (a.bci[mid] < cstart)
+                        lo4[u] = mid + 1;
+                    else
+                        hi4[u] = mid;
+                }
+            }
+            if (!any) break;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int ap = base + u * 64 + lane;
+            if (ap < ae) cur_a[ap - as] = make_int2(lo4[u], e4[u]);
         }
     }
 

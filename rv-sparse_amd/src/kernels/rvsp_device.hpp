@@ -43,6 +43,7 @@ constexpr int kChunkWords = kChunkCols / 32;
 constexpr int kBigPW = 4096;              // outputs per big-row window
 constexpr int kBigMaxChunks = 32;         // chunks per big-row window
 constexpr int kBigMaxWords = kBigMaxChunks * kChunkWords;
+constexpr int kSegProducts = 65536;       // products per big-row segment (one wave)
 constexpr int kDenseCols = 16384;         // single dense window (64 KB acc)
 constexpr int kHashMaxNnz = 2048;         // numeric hash rows: nnz(C_i) <= this
 constexpr int kSymHashMaxU = 2048;        // symbolic hash rows: min(ops,span) <= this

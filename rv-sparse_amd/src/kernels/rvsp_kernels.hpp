@@ -24,6 +24,7 @@ struct AnalysisCounters {
 struct BigArgs {
     const int32_t *arp, *aci, *brp, *bci;
     const int32_t *lo, *hi;
+    const int64_t *ops;
     int32_t *cnt;
     int64_t *big_off;     // per row: offset of its bitmap words in gbits
     uint32_t *gbits;

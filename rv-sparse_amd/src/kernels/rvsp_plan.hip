@@ -296,6 +296,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->big.bci = bci;
     p->big.lo = p->lo;
     p->big.hi = p->hi;
+    p->big.ops = p->ops;
     p->big.cnt = p->cnt;
     p->big.bits_fill = &p->ctr->bits_fill;
     if (nbig > 0) {
