@@ -119,6 +119,27 @@ rvsp_status_t rvsp_spgemm_csr_device_algo(const rvsp_csr_matrix_t *A,
                                           rvsp_csr_matrix_t *C,
                                           rvsp_spgemm_algo_t algo);
 
+/* ---- batched products ---------------------------------------------------- */
+
+typedef struct rvsp_spgemm_batch *rvsp_spgemm_batch_t;
+
+/*
+ * C[i] = A[i] * B[i] for i < count, all device-resident FP32 CSR, computed
+ * as one stacked product per chunk (chunks keep each stacked C within
+ * int32 indexing), so every kernel launch covers many problems. Results are
+ * bit-identical to separate rvsp_spgemm_csr_device calls. C[i] receives
+ * device pointers into storage owned by *batch (owns_data = 0); release it
+ * with rvsp_spgemm_batch_destroy. Synchronous.
+ */
+rvsp_status_t rvsp_spgemm_batched_device(int32_t count, const rvsp_csr_matrix_t *A,
+                                         const rvsp_csr_matrix_t *B, rvsp_csr_matrix_t *C,
+                                         rvsp_spgemm_algo_t algo, rvsp_spgemm_batch_t *batch);
+
+/* Number of stacked products (chunks) the batch was computed in. */
+int32_t rvsp_spgemm_batch_chunks(rvsp_spgemm_batch_t batch);
+
+void rvsp_spgemm_batch_destroy(rvsp_spgemm_batch_t batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,33 @@ rvsp_status_t rvsp_hip_copy_async(void *dst, const void *src, size_t bytes,
 rvsp_status_t rvsp_hip_alloc_async(void **ptr, size_t bytes, void *stream);
 rvsp_status_t rvsp_hip_free_async(void *ptr, void *stream);
 
+/* Batch packing (see rvsp_batch.hip). One item per stacked matrix: its
+ * device arrays, its first row and first entry in the packed arrays, and
+ * the shift added to its column indices. */
+typedef struct
+{
+    const int32_t *row_ptr;
+    const int32_t *col_idx;
+    const float *values;
+    int32_t rows;
+    int32_t nnz;
+    int64_t row_off;
+    int64_t nnz_off;
+    int32_t col_shift;
+} rvsp_hip_pack_item_t;
+
+/* Writes the stacked row pointer (rows+1 entries in total), column indices
+ * and values of `count` matrices into out_*. Synchronises the stream. */
+rvsp_status_t rvsp_hip_pack(void *stream, int32_t count, const rvsp_hip_pack_item_t *items,
+                            int32_t *out_rp, int32_t *out_ci, float *out_va);
+
+/* Splits a stacked row pointer back into per-problem local row pointers,
+ * written consecutively to out_rp (rows[i]+1 entries each), and returns
+ * each problem's first entry in the stacked arrays in bases_host. */
+rvsp_status_t rvsp_hip_unpack_row_ptr(void *stream, int32_t count, const int64_t *row_off,
+                                      const int32_t *rows, const int32_t *big_rp,
+                                      int32_t *out_rp, int64_t *bases_host);
+
 /* Test hook: makes the next n device allocations fail, so cleanup paths
  * can be exercised without exhausting memory. */
 void rvsp_hip_fail_next_allocs(int n);

@@ -119,6 +119,10 @@ def _declare(lib):
         "rvsp_hip_alloc_async": (C.c_int, [P(V), SZ, V]),
         "rvsp_hip_free_async": (C.c_int, [V, V]),
         "rvsp_hip_stream_sync": (C.c_int, [V]),
+        "rvsp_hip_copy_async": (C.c_int, [V, V, SZ, V]),
+        "rvsp_spgemm_batched_device": (C.c_int, [I32, CSRP, CSRP, CSRP, C.c_int, P(V)]),
+        "rvsp_spgemm_batch_chunks": (I32, [V]),
+        "rvsp_spgemm_batch_destroy": (None, [V]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -399,6 +403,23 @@ def get_timing():
     calls = (C.c_longlong * 4)()
     check("get_timing", lib().rvsp_hip_get_timing(ms, calls, 4))
     return {name: (ms[i], calls[i]) for i, name in enumerate(PHASES)}
+
+
+def spgemm_batched(dAs, dBs, algo=ALGO_DEFAULT, fetch=True, with_nnz=False):
+    """rvsp_spgemm_batched_device over lists of DeviceCsr. Returns
+    (list of host Csr or None, number of chunks[, nnz per problem]); the
+    batch is released."""
+    n = len(dAs)
+    A = (CsrStruct * n)(*[d.struct() for d in dAs])
+    B = (CsrStruct * n)(*[d.struct() for d in dBs])
+    Cs = (CsrStruct * n)()
+    h = C.c_void_p()
+    check("batched", lib().rvsp_spgemm_batched_device(n, A, B, Cs, algo, C.byref(h)))
+    chunks = lib().rvsp_spgemm_batch_chunks(h)
+    out = [device_csr_to_host(Cs[i]) for i in range(n)] if fetch else None
+    nnzs = [Cs[i].nnz for i in range(n)]
+    lib().rvsp_spgemm_batch_destroy(h)
+    return (out, chunks, nnzs) if with_nnz else (out, chunks)
 
 
 class Event:

@@ -309,3 +309,25 @@ def test_numeric_is_deterministic():
     c1 = rvsp.spgemm_raw(A, B)
     c2 = rvsp.spgemm_raw(A, B)
     np.testing.assert_array_equal(c1.values.view(np.uint32), c2.values.view(np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# batched products: one stacked launch set, identical to separate calls
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("chunk_products", [None, "20000"])
+def test_batched_matches_separate_calls(chunk_products, monkeypatch):
+    if chunk_products:
+        monkeypatch.setenv("RVSP_BATCH_CHUNK_PRODUCTS", chunk_products)
+    pairs = W.batched(count=9, lo=64, hi=1500, seed=21)
+    dA = [rvsp.DeviceCsr(A) for A, _ in pairs]
+    dB = [rvsp.DeviceCsr(B) for _, B in pairs]
+    got, chunks = rvsp.spgemm_batched(dA, dB)
+    say(f"batched: {len(pairs)} problems in {chunks} chunk(s)")
+    if chunk_products:
+        assert chunks > 1
+    for i, ((A, B), Ci) in enumerate(zip(pairs, got)):
+        single = rvsp.spgemm_device(dA[i], dB[i])
+        np.testing.assert_array_equal(Ci.row_ptr, single.row_ptr)
+        np.testing.assert_array_equal(Ci.col_idx, single.col_idx)
+        np.testing.assert_array_equal(Ci.values.view(np.uint32), single.values.view(np.uint32))
+        check_product(A, B, Ci, f"batched[{i}]")
