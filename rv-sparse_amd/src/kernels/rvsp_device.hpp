@@ -27,14 +27,15 @@ enum SymBin : int {
     SB_COUNT
 };
 
-// Numeric bins: 0 empty, 1 + 4*(log2(H)-6) + (log2(G)-3) for LDS hash
-// tables of H slots worked by G lanes, then a single dense window over the
-// row span, then column windows with cursors.
+// Numeric bins: 0 empty; NB_HASH0 + 4*(log2(H)-6) + (log2(G)-3) for LDS
+// hash tables of H slots worked by G lanes; NB_DENSE0 + 4*(log2(W)-9) +
+// (log2(G)-3) for dense windows of W columns over the row's product span;
+// NB_WIN for big rows (rvsp_bigrow.hip).
 constexpr int NB_EMPTY = 0;
 constexpr int NB_HASH0 = 1;
-constexpr int NB_DENSE = 1 + 7 * 4;  // 29
-constexpr int NB_WIN = NB_DENSE + 1; // 30
-constexpr int NB_COUNT = NB_WIN + 1; // 31
+constexpr int NB_DENSE0 = NB_HASH0 + 7 * 4;  // 29
+constexpr int NB_WIN = NB_DENSE0 + 6 * 4;    // 53
+constexpr int NB_COUNT = NB_WIN + 1;         // 54
 
 constexpr int kBmsWords = 512;            // 16384 columns
 constexpr int kBigPassWords = 16384;      // big-row bitmap: 2^19 columns per pass
@@ -44,8 +45,8 @@ constexpr int kBigPW = 4096;              // outputs per big-row window
 constexpr int kBigMaxChunks = 32;         // chunks per big-row window
 constexpr int kBigMaxWords = kBigMaxChunks * kChunkWords;
 constexpr int kSegProducts = 65536;       // products per big-row segment (one wave)
-constexpr int kDenseCols = 16384;         // single dense window (64 KB acc)
-constexpr int kHashMaxNnz = 2048;         // numeric hash rows: nnz(C_i) <= this
+constexpr int kDenseMaxCols = 16384;      // widest dense window (64 KB acc)
+constexpr int kHashMaxNnz = 2730;         // numeric hash rows: H = 4096 at load <= 2/3
 constexpr int kSymHashMaxU = 2048;        // symbolic hash rows: min(ops,span) <= this
 
 __device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }

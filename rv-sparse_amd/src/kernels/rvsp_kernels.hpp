@@ -10,7 +10,7 @@ namespace rvsp {
 // analysis phase.
 struct AnalysisCounters {
     int sym_hist[16];
-    int num_hist[32];
+    int num_hist[64];
     unsigned long long products;
     unsigned long long bits_cap;   // sum over big rows of bitmap words
     unsigned long long bits_fill;  // running allocation of bitmap words
@@ -72,8 +72,8 @@ void launch_scan(hipStream_t st, int32_t m, const int32_t *cnt, int64_t *tsum, i
                  int32_t *crp);
 int scan_tiles(int32_t m);
 void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_t *ops,
-                     const int32_t *arp, const uint8_t *sbin, uint8_t *nbin,
-                     AnalysisCounters *ctr);
+                     const int32_t *arp, const uint8_t *sbin, const int32_t *lo,
+                     const int32_t *hi, uint8_t *nbin, AnalysisCounters *ctr);
 void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
                     const int32_t *off, const BigArgs &big, int nsegs);
 void launch_sym_big(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g);

@@ -16,7 +16,8 @@
 // Rows are binned (exact nnz(C_i) from the symbolic pass):
 //   hash    nnz(C_i) <= 2048: LDS table of H >= 2 nnz slots, pre-loaded with
 //           C_i's columns, G lanes per row (G ~ mean |B_k|)
-//   dense   one dense LDS window over the row's span (<= 16384 columns)
+//   dense   an LDS window of W >= span columns indexed by column, when
+//           it is at most twice the hash table's bytes
 //   big     rows the symbolic pass kept a bitmap for (rvsp_bigrow.hip)
 #include "rvsp_device.hpp"
 #include "rvsp_kernels.hpp"
@@ -76,51 +77,55 @@ __global__ __launch_bounds__(R *G) void k_num_hash(const int32_t *__restrict__ r
     }
 }
 
-// One wave per row whose product columns span at most WCOLS columns: a
-// dense LDS accumulator over [lo, lo + WCOLS).
-template <int WCOLS>
-__global__ __launch_bounds__(64) void k_num_dense(const int32_t *__restrict__ rows, int32_t nrows,
-                                                  NumArgs a)
+// Dense window: G lanes per row, R rows per block, an LDS accumulator of W
+// columns over [lo_i, lo_i + W) indexed directly by column. Same ordered
+// walk as the hash kernel, no probing; drained through C_i's columns.
+template <int G, int W, int R>
+__global__ __launch_bounds__(R *G) void k_num_dense(const int32_t *__restrict__ rows, int32_t nrows,
+                                                    NumArgs a)
 {
-    __shared__ float S[WCOLS];
-    const int lane = threadIdx.x;
-    const int row = rows[blockIdx.x];
-    for (int i = lane; i < WCOLS; i += 64) S[i] = 0.0f;
-    const int32_t wlo = a.lo[row];
-    wave_sync();
-    const int as = a.arp[row], ae = a.arp[row + 1];
-    for (int base = as; base < ae; base += 64) {
-        const int ap = base + lane;
-        float av = 0.0f;
-        int b0 = 0, b1 = 0;
-        if (ap < ae) {
-            av = a.aval[ap];
-            const int k = a.aci[ap];
-            b0 = a.brp[k];
-            b1 = a.brp[k + 1];
-        }
-        uint64_t live = __ballot(b1 > b0);
-        while (live) {
-            const int j = __ffsll((unsigned long long)live) - 1;
-            live &= live - 1;
-            const float aj = __shfl(av, j, 64);
-            const int sj = __shfl(b0, j, 64), ej = __shfl(b1, j, 64);
-            for (int bp = sj + lane; bp < ej; bp += 64) {
-                const int32_t c = a.bci[bp] - wlo;
-                const float prod = __fmul_rn(aj, a.bval[bp]);
-                if ((uint32_t)c < (uint32_t)WCOLS)
-                    atomicAdd(&S[c], prod);
-                else
-                    atomicOr(a.err, kErrWindow);
+    __shared__ float accs[R * W];
+    const int g = threadIdx.x / G, lane = threadIdx.x % G;
+    const int64_t ridx = (int64_t)blockIdx.x * R + g;
+    const bool valid = ridx < nrows;
+    const int row = valid ? rows[ridx] : 0;
+    float *S = accs + g * W;
+    for (int i = lane; i < W; i += G) S[i] = 0.0f;
+    __syncthreads();
+    const int32_t wlo = valid ? a.lo[row] : 0;
+    if (valid) {
+        const int as = a.arp[row], ae = a.arp[row + 1];
+        for (int base = as; base < ae; base += G) {
+            const int ap = base + lane;
+            float av = 0.0f;
+            int bs = 0, be = 0;
+            if (ap < ae) {
+                const int k = a.aci[ap];
+                av = a.aval[ap];
+                bs = a.brp[k];
+                be = a.brp[k + 1];
             }
-            compiler_fence();
+            const int nch = min(G, ae - base);
+            for (int j = 0; j < nch; ++j) {
+                const float aj = __shfl(av, j, G);
+                const int sj = __shfl(bs, j, G), ej = __shfl(be, j, G);
+                for (int bp = sj + lane; bp < ej; bp += G) {
+                    const int32_t c = a.bci[bp] - wlo;
+                    const float prod = __fmul_rn(aj, a.bval[bp]);
+                    if ((uint32_t)c < (uint32_t)W)
+                        atomicAdd(&S[c], prod);
+                    else
+                        atomicOr(a.err, kErrWindow);
+                }
+                compiler_fence();
+            }
         }
     }
-    wave_sync();
-    const int cs = a.crp[row], ce = a.crp[row + 1];
-    for (int q = cs + lane; q < ce; q += 64) {
+    __syncthreads();
+    const int cs = valid ? a.crp[row] : 0, ce = valid ? a.crp[row + 1] : 0;
+    for (int q = cs + lane; q < ce; q += G) {
         const int32_t c = a.cci[q] - wlo;
-        a.cval[q] = (uint32_t)c < (uint32_t)WCOLS ? S[c] : 0.0f;
+        a.cval[q] = (uint32_t)c < (uint32_t)W ? S[c] : 0.0f;
     }
 }
 
@@ -133,9 +138,22 @@ static void launch_hash(hipStream_t st, const NumArgs &a, const int32_t *rows, i
     k_num_hash<G, H, R><<<grid, R * G, 0, st>>>(rows, nrows, a);
 }
 
+template <int G, int W>
+static void launch_dense(hipStream_t st, const NumArgs &a, const int32_t *rows, int nrows)
+{
+    constexpr int R = (256 / G) < (16384 / W) ? (256 / G) : (16384 / W);
+    static_assert(R * G >= 64, "block must hold at least one full wave");
+    const int grid = (nrows + R - 1) / R;
+    k_num_dense<G, W, R><<<grid, R * G, 0, st>>>(rows, nrows, a);
+}
+
 #define RVSP_HASH_CASE(HI, GI, H, G)                                                            \
     case NB_HASH0 + 4 * (HI) + (GI):                                                            \
         launch_hash<G, H>(st, a, rows, nr);                                                     \
+        break;
+#define RVSP_DENSE_CASE(WI, GI, W, G)                                                           \
+    case NB_DENSE0 + 4 * (WI) + (GI):                                                           \
+        launch_dense<G, W>(st, a, rows, nr);                                                    \
         break;
 
 void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
@@ -171,9 +189,24 @@ void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const
             RVSP_HASH_CASE(5, 3, 2048, 64)
             RVSP_HASH_CASE(6, 2, 4096, 32)
             RVSP_HASH_CASE(6, 3, 4096, 64)
-        case NB_DENSE:
-            k_num_dense<kDenseCols><<<nr, 64, 0, st>>>(rows, nr, a);
-            break;
+            RVSP_DENSE_CASE(0, 0, 512, 8)
+            RVSP_DENSE_CASE(0, 1, 512, 16)
+            RVSP_DENSE_CASE(0, 2, 512, 32)
+            RVSP_DENSE_CASE(0, 3, 512, 64)
+            RVSP_DENSE_CASE(1, 0, 1024, 8)
+            RVSP_DENSE_CASE(1, 1, 1024, 16)
+            RVSP_DENSE_CASE(1, 2, 1024, 32)
+            RVSP_DENSE_CASE(1, 3, 1024, 64)
+            RVSP_DENSE_CASE(2, 0, 2048, 8)
+            RVSP_DENSE_CASE(2, 1, 2048, 16)
+            RVSP_DENSE_CASE(2, 2, 2048, 32)
+            RVSP_DENSE_CASE(2, 3, 2048, 64)
+            RVSP_DENSE_CASE(3, 1, 4096, 16)
+            RVSP_DENSE_CASE(3, 2, 4096, 32)
+            RVSP_DENSE_CASE(3, 3, 4096, 64)
+            RVSP_DENSE_CASE(4, 2, 8192, 32)
+            RVSP_DENSE_CASE(4, 3, 8192, 64)
+            RVSP_DENSE_CASE(5, 3, 16384, 64)
         default:
             break;
         }

@@ -341,7 +341,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
         launch_big_plan(st, p->sperm + p->sym_off[SB_BIG], nbig, p->big, p->crp, p->ctr);
         HIP_TRY(hipGetLastError());
     }
-    launch_num_bins(st, m, p->crp, p->ops, arp, p->sbin, p->nbin, p->ctr);
+    launch_num_bins(st, m, p->crp, p->ops, arp, p->sbin, p->lo, p->hi, p->nbin, p->ctr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&p->hctr, p->ctr, sizeof(AnalysisCounters), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));

@@ -333,15 +333,29 @@ __global__ __launch_bounds__(256) void k_scan_apply(int32_t m, const int32_t *__
 // mean B-row length), a single dense window, or the big-row path (rows the
 // symbolic pass kept a bitmap for).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int num_hash_bin(int nz, int64_t ops, int alen)
+// Lanes per row: about the mean |B_k| (lanes cover one B row per k-step),
+// raised so that a block of R rows still holds a full wave.
+__device__ __forceinline__ int lanes_for(int64_t ops, int alen, int rows_per_block_max)
 {
-    const int H = max(64, next_pow2(2 * nz));
     const int64_t avgb = alen > 0 ? (ops + alen - 1) / alen : 1;
     int G = next_pow2((int)min<int64_t>(avgb, 64));
     G = max(8, min(64, G));
-    const int rmax = 8192 / H;  // rows per block so that 8*H*R <= 64 KB
-    const int gmin = rmax >= 8 ? 8 : 64 / rmax;
-    G = max(G, gmin);
+    const int gmin = rows_per_block_max >= 8 ? 8 : 64 / rows_per_block_max;
+    return max(G, gmin);
+}
+
+// Hash table of H >= 1.5 nnz slots (8 bytes each) or a dense window of
+// W >= span columns (4 bytes each): the dense window wins when it is at
+// most twice the table's bytes, since it needs no probing at all.
+__device__ __forceinline__ int num_bin_of(int nz, int64_t ops, int alen, int64_t span)
+{
+    const int H = nz <= kHashMaxNnz ? max(64, next_pow2((3 * nz + 1) / 2)) : 0;
+    const int Wd = span <= kDenseMaxCols ? max(512, next_pow2((int)span)) : 0;
+    if (Wd && (!H || 4 * Wd <= 2 * 8 * H)) {
+        const int G = lanes_for(ops, alen, 16384 / Wd);
+        return NB_DENSE0 + 4 * (ilog2(Wd) - 9) + (ilog2(G) - 3);
+    }
+    const int G = lanes_for(ops, alen, 8192 / H);
     return NB_HASH0 + 4 * (ilog2(H) - 6) + (ilog2(G) - 3);
 }
 
@@ -349,6 +363,8 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
                                                   const int64_t *__restrict__ ops,
                                                   const int32_t *__restrict__ arp,
                                                   const uint8_t *__restrict__ sbin,
+                                                  const int32_t *__restrict__ lo,
+                                                  const int32_t *__restrict__ hi,
                                                   uint8_t *__restrict__ nbin,
                                                   AnalysisCounters *__restrict__ ctr)
 {
@@ -363,10 +379,8 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
             b = NB_EMPTY;
         else if (sbin[i] == SB_BIG)
             b = NB_WIN;  // segments planned by k_big_plan
-        else if (nz <= kHashMaxNnz)
-            b = num_hash_bin(nz, ops[i], arp[i + 1] - arp[i]);
         else
-            b = NB_DENSE;
+            b = num_bin_of(nz, ops[i], arp[i + 1] - arp[i], (int64_t)hi[i] - lo[i] + 1);
         nbin[i] = (uint8_t)b;
         atomicAdd(&hist[b], 1);
     }
@@ -464,11 +478,11 @@ int scan_tiles(int32_t m)
 }
 
 void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_t *ops,
-                     const int32_t *arp, const uint8_t *sbin, uint8_t *nbin,
-                     AnalysisCounters *ctr)
+                     const int32_t *arp, const uint8_t *sbin, const int32_t *lo,
+                     const int32_t *hi, uint8_t *nbin, AnalysisCounters *ctr)
 {
     const int grid = (int)(((int64_t)m + 255) / 256);
-    k_num_bins<<<grid, 256, 0, st>>>(m, crp, ops, arp, sbin, nbin, ctr);
+    k_num_bins<<<grid, 256, 0, st>>>(m, crp, ops, arp, sbin, lo, hi, nbin, ctr);
 }
 
 } // namespace rvsp
