@@ -36,6 +36,8 @@ constexpr int NB_HASH0 = 1;
 constexpr int NB_DENSE0 = NB_HASH0 + 7 * 4;  // 29
 constexpr int NB_WIN = NB_DENSE0 + 6 * 4;    // 53
 constexpr int NB_COUNT = NB_WIN + 1;         // 54
+// bin scatter keeps per-bin offsets and cursors in 64-entry regions
+static_assert(NB_COUNT <= 64 && SB_COUNT <= 64, "bin tables hold 64 bins");
 
 constexpr int kBmsWords = 512;            // 16384 columns
 constexpr int kBigPassWords = 16384;      // big-row bitmap: 2^19 columns per pass

@@ -133,7 +133,7 @@ struct rvsp_hip_plan {
     int32_t *cnt = nullptr, *crp = nullptr, *sperm = nullptr, *nperm = nullptr;
     int64_t *tsum = nullptr;
     AnalysisCounters *ctr = nullptr;
-    int32_t *binoff = nullptr;  // [0,32) offsets, [32,64) cursors
+    int32_t *binoff = nullptr;  // [0,64) bin offsets, [64,128) scatter cursors
     int *err = nullptr;
 
     void *bigblock = nullptr;  // big-row bitmaps, chunk counts, windows, segments
@@ -243,7 +243,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     const size_t o_nperm = off; off = align256(off + M * 4);
     const size_t o_tsum = off; off = align256(off + (size_t)ntiles * 8);
     const size_t o_ctr = off; off = align256(off + sizeof(AnalysisCounters));
-    const size_t o_binoff = off; off = align256(off + 64 * 4);
+    const size_t o_binoff = off; off = align256(off + 128 * 4);
     const size_t o_err = off; off = align256(off + 4);
     rvsp_status_t s = dalloc(&p->rowblock, off, st);
     if (s != RVSP_SUCCESS) return s;
@@ -264,7 +264,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
 
     PhaseScope timer(st, PH_ANALYSE);
     HIP_TRY(hipMemsetAsync(p->ctr, 0, sizeof(AnalysisCounters), st));
-    HIP_TRY(hipMemsetAsync(p->binoff, 0, 64 * 4, st));
+    HIP_TRY(hipMemsetAsync(p->binoff, 0, 128 * 4, st));
     HIP_TRY(hipMemsetAsync(p->err, 0, 4, st));
 
     // ---- 1. per-row product counts, spans and symbolic bins
@@ -285,7 +285,8 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
         }
     }
     HIP_TRY(hipMemcpyAsync(p->binoff, hoff, 64 * 4, hipMemcpyHostToDevice, st));
-    launch_bin_scatter(st, m, p->sbin, p->binoff, p->binoff + 32, p->sperm, SB_COUNT);
+    HIP_TRY(hipMemsetAsync(p->binoff + 64, 0, 64 * 4, st));
+    launch_bin_scatter(st, m, p->sbin, p->binoff, p->binoff + 64, p->sperm, SB_COUNT);
     HIP_TRY(hipGetLastError());
 
     const int nbig = p->sym_count[SB_BIG];
@@ -366,7 +367,8 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
         }
     }
     HIP_TRY(hipMemcpyAsync(p->binoff, hoff, 64 * 4, hipMemcpyHostToDevice, st));
-    launch_bin_scatter(st, m, p->nbin, p->binoff, p->binoff + 32, p->nperm, NB_COUNT);
+    HIP_TRY(hipMemsetAsync(p->binoff + 64, 0, 64 * 4, st));
+    launch_bin_scatter(st, m, p->nbin, p->binoff, p->binoff + 64, p->nperm, NB_COUNT);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
     int herr = 0;
