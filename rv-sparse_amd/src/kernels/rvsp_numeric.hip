@@ -24,6 +24,54 @@
 
 namespace rvsp {
 
+// Walks one row's A entries in stored order with G lanes over each B_k and
+// calls add(col, fl(a*b)) for every product. Loads for four k-steps are
+// issued before any of their adds, so up to eight loads per lane are in
+// flight; the adds of each k-step are then issued in k order, separated by
+// compiler fences (B is read-only, so hoisting its loads is safe).
+template <int G, typename Add>
+__device__ __forceinline__ void walk_row_ordered(const NumArgs &a, int as, int ae, Add add)
+{
+    const int lane = threadIdx.x % G;
+    for (int base = as; base < ae; base += G) {
+        const int ap = base + lane;
+        float av = 0.0f;
+        int bs = 0, be = 0;
+        if (ap < ae) {
+            const int k = a.aci[ap];
+            av = a.aval[ap];
+            bs = a.brp[k];
+            be = a.brp[k + 1];
+        }
+        const int nch = min(G, ae - base);
+        for (int j0 = 0; j0 < nch; j0 += 4) {
+            float au[4], vu[4];
+            int su[4], eu[4];
+            int32_t cu[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = min(j0 + u, G - 1);
+                au[u] = __shfl(av, j, G);
+                su[u] = __shfl(bs, j, G);
+                eu[u] = __shfl(be, j, G);
+                const int bp = su[u] + lane;
+                const bool ok = j0 + u < nch && bp < eu[u];
+                cu[u] = ok ? a.bci[bp] : -1;
+                vu[u] = ok ? a.bval[bp] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (j0 + u < nch) {  // uniform within the group
+                    if (cu[u] >= 0) add(cu[u], __fmul_rn(au[u], vu[u]));
+                    for (int bp = su[u] + G + lane; bp < eu[u]; bp += G)
+                        add(a.bci[bp], __fmul_rn(au[u], a.bval[bp]));
+                    compiler_fence();
+                }
+            }
+        }
+    }
+}
+
 template <int G, int H, int R>
 __global__ __launch_bounds__(R *G) void k_num_hash(const int32_t *__restrict__ rows, int32_t nrows,
                                                    NumArgs a)
@@ -45,30 +93,11 @@ __global__ __launch_bounds__(R *G) void k_num_hash(const int32_t *__restrict__ r
     for (int q = cs + lane; q < ce; q += G) hash_insert_slot<H>(T, a.cci[q], a.err);
     __syncthreads();
     if (valid) {
-        const int as = a.arp[row], ae = a.arp[row + 1];
-        for (int base = as; base < ae; base += G) {
-            const int ap = base + lane;
-            float av = 0.0f;
-            int bs = 0, be = 0;
-            if (ap < ae) {
-                const int k = a.aci[ap];
-                av = a.aval[ap];
-                bs = a.brp[k];
-                be = a.brp[k + 1];
-            }
-            const int nch = min(G, ae - base);
-            for (int j = 0; j < nch; ++j) {
-                const float aj = __shfl(av, j, G);
-                const int sj = __shfl(bs, j, G), ej = __shfl(be, j, G);
-                for (int bp = sj + lane; bp < ej; bp += G) {
-                    const int32_t col = a.bci[bp];
-                    const float prod = __fmul_rn(aj, a.bval[bp]);
-                    const int s = hash_find<H>(T, col, a.err);
-                    if (s >= 0) atomicAdd(&V[s], prod);
-                }
-                compiler_fence();
-            }
-        }
+        int *err = a.err;
+        walk_row_ordered<G>(a, a.arp[row], a.arp[row + 1], [&](int32_t col, float prod) {
+            const int s = hash_find<H>(T, col, err);
+            if (s >= 0) atomicAdd(&V[s], prod);
+        });
     }
     __syncthreads();
     for (int q = cs + lane; q < ce; q += G) {
@@ -94,32 +123,14 @@ __global__ __launch_bounds__(R *G) void k_num_dense(const int32_t *__restrict__ 
     __syncthreads();
     const int32_t wlo = valid ? a.lo[row] : 0;
     if (valid) {
-        const int as = a.arp[row], ae = a.arp[row + 1];
-        for (int base = as; base < ae; base += G) {
-            const int ap = base + lane;
-            float av = 0.0f;
-            int bs = 0, be = 0;
-            if (ap < ae) {
-                const int k = a.aci[ap];
-                av = a.aval[ap];
-                bs = a.brp[k];
-                be = a.brp[k + 1];
-            }
-            const int nch = min(G, ae - base);
-            for (int j = 0; j < nch; ++j) {
-                const float aj = __shfl(av, j, G);
-                const int sj = __shfl(bs, j, G), ej = __shfl(be, j, G);
-                for (int bp = sj + lane; bp < ej; bp += G) {
-                    const int32_t c = a.bci[bp] - wlo;
-                    const float prod = __fmul_rn(aj, a.bval[bp]);
-                    if ((uint32_t)c < (uint32_t)W)
-                        atomicAdd(&S[c], prod);
-                    else
-                        atomicOr(a.err, kErrWindow);
-                }
-                compiler_fence();
-            }
-        }
+        int *err = a.err;
+        walk_row_ordered<G>(a, a.arp[row], a.arp[row + 1], [&](int32_t col, float prod) {
+            const int32_t c = col - wlo;
+            if ((uint32_t)c < (uint32_t)W)
+                atomicAdd(&S[c], prod);
+            else
+                atomicOr(err, kErrWindow);
+        });
     }
     __syncthreads();
     const int cs = valid ? a.crp[row] : 0, ce = valid ? a.crp[row + 1] : 0;
