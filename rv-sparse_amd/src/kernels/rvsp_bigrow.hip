@@ -55,34 +55,63 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
         const int32_t pcols = pwn * 32;
         for (int w = tid; w < pwn; w += NT) bits[w] = 0u;
         __syncthreads();
-        for (int ap = as; ap < ae; ++ap) {
-            const int k = g.aci[ap];
-            const int bs = g.brp[k], be = g.brp[k + 1];
-            // 4 entries per thread in flight, then OR-combine each slot
-            for (int base = bs; base < be; base += 4 * NT) {
-                int32_t col[4];
+        // A entries in chunks of NT: with short B rows each thread walks its
+        // own B row (order does not matter for the structure); with long
+        // ones the block covers one B row at a time.
+        for (int cb = as; cb < ae; cb += NT) {
+            const int myap = cb + tid;
+            int mbs = 0, mbe = 0;
+            if (myap < ae) {
+                const int k = g.aci[myap];
+                mbs = g.brp[k];
+                mbe = g.brp[k + 1];
+            }
+            const int cnt = min(NT, ae - cb);
+            int len = mbe - mbs;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int bp = base + u * NT + tid;
-                    col[u] = bp < be ? g.bci[bp] : -1;
+            for (int o = 32; o > 0; o >>= 1) len += __shfl_xor(len, o, 64);
+            if (lane == 0) s_part[wv] = len;
+            __syncthreads();
+            int tot = 0;
+#pragma unroll
+            for (int i = 0; i < NT / 64; ++i) tot += s_part[i];
+            __syncthreads();
+            if (tot < 64 * cnt) {
+                for (int bp = mbs; bp < mbe; ++bp) {
+                    const int32_t c = g.bci[bp] - base_col;
+                    if (c >= 0 && c < pcols) atomicOr(&bits[c >> 5], 1u << (c & 31));
                 }
+                continue;
+            }
+            for (int ap = cb; ap < cb + cnt; ++ap) {
+                const int k = g.aci[ap];
+                const int bs = g.brp[k], be = g.brp[k + 1];
+                // 4 entries per thread in flight, then OR-combine each slot
+                for (int base = bs; base < be; base += 4 * NT) {
+                    int32_t col[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (base + u * NT >= be) break;  // block-uniform
-                    const bool valid = base + u * NT + tid < be;
-                    const int32_t c = col[u] - base_col;
-                    const bool in = valid && c >= 0 && c < pcols;
-                    int wd = in ? (c >> 5) : -1 - lane;
-                    uint32_t wb = in ? (1u << (c & 31)) : 0u;
-#pragma unroll
-                    for (int o = 1; o < 64; o <<= 1) {
-                        const uint32_t ob = __shfl_up(wb, o, 64);
-                        const int ow = __shfl_up(wd, o, 64);
-                        if (lane >= o && ow == wd) wb |= ob;
+                    for (int u = 0; u < 4; ++u) {
+                        const int bp = base + u * NT + tid;
+                        col[u] = bp < be ? g.bci[bp] : -1;
                     }
-                    const int nwd = __shfl_down(wd, 1, 64);
-                    const bool last = lane == 63 || nwd != wd;
-                    if (in && last) atomicOr(&bits[wd], wb);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (base + u * NT >= be) break;  // block-uniform
+                        const bool valid = base + u * NT + tid < be;
+                        const int32_t c = col[u] - base_col;
+                        const bool in = valid && c >= 0 && c < pcols;
+                        int wd = in ? (c >> 5) : -1 - lane;
+                        uint32_t wb = in ? (1u << (c & 31)) : 0u;
+#pragma unroll
+                        for (int o = 1; o < 64; o <<= 1) {
+                            const uint32_t ob = __shfl_up(wb, o, 64);
+                            const int ow = __shfl_up(wd, o, 64);
+                            if (lane >= o && ow == wd) wb |= ob;
+                        }
+                        const int nwd = __shfl_down(wd, 1, 64);
+                        const bool last = lane == 63 || nwd != wd;
+                        if (in && last) atomicOr(&bits[wd], wb);
+                    }
                 }
             }
         }
