@@ -136,13 +136,19 @@ def cpu_baseline(A, B, target_s=10.0, seed=11):
     }
 
 
+ALGOS = {"hip": rvsp.ALGO_HIP, "learned": rvsp.ALGO_HIP_LEARNED}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rmat")
-    ap.add_argument("--algo", default="hip", choices=["hip", "learned"])
+    ap.add_argument("--algo", default="learned", choices=["hip", "learned"],
+                    help="learned: MLP row selectors (the plain heuristic is timed beside it)")
+    ap.add_argument("--selector-eval", action="store_true",
+                    help="also time heuristic vs learned vs exhaustive-best numeric bins")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
@@ -162,13 +168,13 @@ def main():
     log(f"workload {args.workload}: {desc}; nnzA={A.nnz} nnzB={B.nnz} ({time.time() - t0:.1f} s)")
     dA = rvsp.DeviceCsr(A)
     dB = dA if same else rvsp.DeviceCsr(B)
-    algo = rvsp.ALGO_HIP if args.algo == "hip" else rvsp.ALGO_HIP_LEARNED
+    algo = ALGOS[args.algo]
     L = rvsp.lib()
     a, b = dA.struct(), dB.struct()
 
-    def step():
+    def step(al=algo):
         c = rvsp.CsrStruct()
-        rvsp.check("csr_device", L.rvsp_spgemm_csr_device_algo(C.byref(a), C.byref(b), C.byref(c), algo))
+        rvsp.check("csr_device", L.rvsp_spgemm_csr_device_algo(C.byref(a), C.byref(b), C.byref(c), al))
         return c
 
     c = step()
@@ -184,22 +190,33 @@ def main():
     products = d.products()
     d.close()
 
-    for _ in range(args.warmup):
-        c = step()
-        L.rvsp_csr_destroy(C.byref(c))
-    rvsp.check("sync", L.rvsp_hip_synchronize())
-    rvsp.reset_timing()
-    rvsp.set_timing(True)
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        c = step()
-        L.rvsp_csr_destroy(C.byref(c))
-        log(f"step {i + 1}/{args.steps}")
-    rvsp.check("sync", L.rvsp_hip_synchronize())
-    t1 = time.perf_counter()
-    rvsp.set_timing(False)
-    phases = rvsp.get_timing()
-    ms_step = (t1 - t0) * 1e3 / args.steps
+    def timed(al, tag):
+        """Warm-up, then exactly args.steps one-shot products between
+        device synchronisations: (ms per step, phase timings)."""
+        for _ in range(args.warmup):
+            c = step(al)
+            L.rvsp_csr_destroy(C.byref(c))
+        rvsp.check("sync", L.rvsp_hip_synchronize())
+        rvsp.reset_timing()
+        rvsp.set_timing(True)
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            c = step(al)
+            L.rvsp_csr_destroy(C.byref(c))
+            log(f"{tag} step {i + 1}/{args.steps}")
+        rvsp.check("sync", L.rvsp_hip_synchronize())
+        t1 = time.perf_counter()
+        rvsp.set_timing(False)
+        return (t1 - t0) * 1e3 / args.steps, rvsp.get_timing()
+
+    other = None
+    if args.algo == "learned":
+        # the plain heuristic-binned path, timed the same way, for comparison
+        ms_plain, ph_plain = timed(ALGOS["hip"], "plain")
+        other = {"algo": "hip", "value": round(2.0 * products / (ms_plain * 1e-3) / 1e9, 3),
+                 "ms_per_step": round(ms_plain, 4),
+                 "phases_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in ph_plain.items()}}
+    ms_step, phases = timed(algo, args.algo)
     gflops = 2.0 * products / (ms_step * 1e-3) / 1e9
 
     num_ms, num_calls = phases["numeric"]
@@ -255,6 +272,13 @@ def main():
         "step_roofline_frac": round(step_gbps / HBM_PEAK_GBPS, 4),
         "cpu_baseline": cpu,
     }
+    if other is not None:
+        out["plain"] = other
+        out["learned_vs_plain"] = round(other["ms_per_step"] / ms_step, 4)
+    if args.selector_eval:
+        from rvsp import selector as SEL
+        log("selector evaluation (heuristic / learned / exhaustive-best numeric bins) ...")
+        out["selector_eval"] = SEL.evaluate_chooser(args.workload, A, B, log=log)
     print(json.dumps(out), flush=True)
 
 
@@ -355,7 +379,7 @@ def bench_batched(args):
     products = sum(int(np.diff(B.row_ptr)[A.col_idx].sum()) for A, B in pairs)
     dA = [rvsp.DeviceCsr(A) for A, _ in pairs]
     dB = [rvsp.DeviceCsr(B) for _, B in pairs]
-    algo = rvsp.ALGO_HIP if args.algo == "hip" else rvsp.ALGO_HIP_LEARNED
+    algo = ALGOS[args.algo]
 
     def separate():
         for i in range(len(pairs)):

@@ -140,6 +140,59 @@ int32_t rvsp_spgemm_batch_chunks(rvsp_spgemm_batch_t batch);
 
 void rvsp_spgemm_batch_destroy(rvsp_spgemm_batch_t batch);
 
+/* ---- learned row selectors ------------------------------------------------ */
+
+/*
+ * RVSP_SPGEMM_ALGO_HIP_LEARNED uses two small MLPs (features -> 16 -> 16 ->
+ * outputs, ReLU) in place of fixed binning rules:
+ *   SIZER   4 features -> 1: predicted log2(nnz(C_i) / min(ops_i, span_i)),
+ *           sizes each row's symbolic LDS hash table (times the margin);
+ *           rows whose exact count overflows are recounted in their
+ *           heuristic bin, so results never depend on the prediction;
+ *   CHOOSER 5 features -> 8: predicted log2(ns per row) of each numeric
+ *           accumulator (LDS hash table or dense window, 8..64 lanes);
+ *           the smallest feasible prediction wins (the margin is unused).
+ * Feature vectors are 5 floats per row: log2(1+alen)/16, log2(1+ops)/32,
+ * log2(1+span)/32, log2(1+ops/alen)/16, log2(1+nnz)/32 (the sizer reads
+ * the first four). Parameters are stored layer by layer: W1[16][nf], b1,
+ * W2[16][16], b2, W3[no][16], b3.
+ * Without loaded parameters the learned algo behaves as RVSP_SPGEMM_ALGO_HIP.
+ * Parameters load on first use from the file named by RVSP_SELECTOR_SIZER /
+ * RVSP_SELECTOR_CHOOSER, else from selector/{sizer,chooser}.txt beside the
+ * library's directory (text: the margin, then the parameters).
+ */
+#define RVSP_SELECTOR_SIZER 0
+#define RVSP_SELECTOR_CHOOSER 1
+#define RVSP_SIZER_PARAMS 369
+#define RVSP_CHOOSER_PARAMS 504
+
+/* Number of parameters of a selector (-1 for an unknown one). */
+int32_t rvsp_selector_param_count(int32_t which);
+/* Installs parameters (host array of param_count floats) and margin > 0. */
+rvsp_status_t rvsp_selector_set(int32_t which, const float *params, int32_t n, float margin);
+rvsp_status_t rvsp_selector_get(int32_t which, float *params, int32_t n, float *margin);
+/* Reads a parameter file (format above) and installs it. */
+rvsp_status_t rvsp_selector_load_file(int32_t which, const char *path);
+/* Turns a selector off (the learned algo then uses the heuristic for it). */
+rvsp_status_t rvsp_selector_clear(int32_t which);
+
+/* Network outputs for n feature rows (device, n x 5) at the given host
+ * parameters: pred (device) receives n x outputs floats. Synchronous. */
+rvsp_status_t rvsp_selector_predict(int32_t which, const float *feats, int32_t n,
+                                    const float *params_host, float *pred);
+/* Gradient at params_host of L = (1/n) * sum over samples and outputs of
+ * (pred - target)^2, NaN targets skipped; feats/target are device arrays
+ * (n x 5, n x outputs). grad_host receives param_count floats, *loss L. */
+rvsp_status_t rvsp_selector_grad(int32_t which, const float *feats, const float *target,
+                                 int32_t n, const float *params_host, float *grad_host,
+                                 float *loss);
+/* Adam (beta 0.9/0.999) on L over minibatches of `batch` samples, reshuffled
+ * each epoch from `seed`; params_host is the start point and receives the
+ * result; *loss_out the last epoch's loss per sample. Runs on the GPU. */
+rvsp_status_t rvsp_selector_train(int32_t which, const float *feats, const float *target,
+                                  int32_t n, int32_t epochs, int32_t batch, float lr,
+                                  uint64_t seed, float *params_host, float *loss_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,36 @@ rvsp_status_t rvsp_hip_plan_numeric_graph(rvsp_hip_plan_t *plan,
  * synchronises the plan stream. */
 rvsp_status_t rvsp_hip_plan_check(rvsp_hip_plan_t *plan);
 
+/* Host copies (m bytes each, either may be NULL) of the per-row symbolic
+ * and numeric bins of the last analysis. Synchronous. */
+rvsp_status_t rvsp_hip_plan_get_bins(rvsp_hip_plan_t *plan, uint8_t *sym_bins,
+                                     uint8_t *num_bins);
+
+/* Replaces the numeric bins (host array, m bytes). Empty and big-row bins
+ * must stay as they are; every other row takes a hash or dense bin, which
+ * must hold the row (else the numeric pass raises the consistency flag).
+ * Drops a captured numeric graph. Synchronous. */
+rvsp_status_t rvsp_hip_plan_set_num_bins(rvsp_hip_plan_t *plan, const uint8_t *num_bins);
+
+/* Selector features of every row (device, m x 5 floats) and the sizer's
+ * target log2(nnz(C_i) / min(ops_i, span_i)) (device, m floats, NaN for
+ * rows without products). Synchronous. */
+rvsp_status_t rvsp_hip_plan_features(rvsp_hip_plan_t *plan, float *feats, float *ratio);
+
+/* Learned sizing of the last analysis: rows kept in a smaller table than
+ * the heuristic's, and rows recounted after an overflow. */
+rvsp_status_t rvsp_hip_plan_selector_stats(const rvsp_hip_plan_t *plan, int64_t *sized_rows,
+                                           int64_t *fallback_rows);
+
+/* Average milliseconds of one numeric launch of hash/dense bin `bin` over
+ * the given device row list (rows may repeat), after one warm-up launch;
+ * writes those rows' values. RVSP_ERROR_INVALID_ARGUMENT if the bin does
+ * not hold a row. Synchronous. */
+rvsp_status_t rvsp_hip_plan_time_bin(rvsp_hip_plan_t *plan, int32_t bin, const int32_t *rows,
+                                     int32_t nrows, const float *a_values,
+                                     const float *b_values, const int32_t *c_col_idx,
+                                     float *c_values, int32_t reps, float *ms);
+
 /* Stream helpers used by the host core. */
 rvsp_status_t rvsp_hip_stream_sync(void *stream);
 rvsp_status_t rvsp_hip_copy_async(void *dst, const void *src, size_t bytes,

@@ -123,6 +123,32 @@ def _declare(lib):
         "rvsp_spgemm_batched_device": (C.c_int, [I32, CSRP, CSRP, CSRP, C.c_int, P(V)]),
         "rvsp_spgemm_batch_chunks": (I32, [V]),
         "rvsp_spgemm_batch_destroy": (None, [V]),
+        # learned selectors (rv_sparse_hip.h)
+        "rvsp_selector_param_count": (I32, [I32]),
+        "rvsp_selector_set": (C.c_int, [I32, V, I32, C.c_float]),
+        "rvsp_selector_get": (C.c_int, [I32, V, I32, P(C.c_float)]),
+        "rvsp_selector_load_file": (C.c_int, [I32, C.c_char_p]),
+        "rvsp_selector_clear": (C.c_int, [I32]),
+        "rvsp_selector_predict": (C.c_int, [I32, V, I32, V, V]),
+        "rvsp_selector_grad": (C.c_int, [I32, V, V, I32, V, V, P(C.c_float)]),
+        "rvsp_selector_train": (C.c_int, [I32, V, V, I32, I32, I32, C.c_float, C.c_uint64, V,
+                                          P(C.c_float)]),
+        # device plan (rvsp_hip_abi.h)
+        "rvsp_hip_plan_create": (C.c_int, [P(V), V]),
+        "rvsp_hip_plan_destroy": (None, [V]),
+        "rvsp_hip_plan_stream": (V, [V]),
+        "rvsp_hip_plan_analyse": (C.c_int, [V, I32, I32, I32, I32, V, V, I32, V, V, C.c_int,
+                                            P(I64), P(I64)]),
+        "rvsp_hip_plan_row_ptr": (V, [V]),
+        "rvsp_hip_plan_fill": (C.c_int, [V, V]),
+        "rvsp_hip_plan_numeric": (C.c_int, [V, V, V, V, V]),
+        "rvsp_hip_plan_check": (C.c_int, [V]),
+        "rvsp_hip_plan_get_bins": (C.c_int, [V, V, V]),
+        "rvsp_hip_plan_set_num_bins": (C.c_int, [V, V]),
+        "rvsp_hip_plan_features": (C.c_int, [V, V, V]),
+        "rvsp_hip_plan_selector_stats": (C.c_int, [V, P(I64), P(I64)]),
+        "rvsp_hip_plan_time_bin": (C.c_int, [V, I32, V, I32, V, V, V, V, I32,
+                                             P(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -420,6 +446,160 @@ def spgemm_batched(dAs, dBs, algo=ALGO_DEFAULT, fetch=True, with_nnz=False):
     nnzs = [Cs[i].nnz for i in range(n)]
     lib().rvsp_spgemm_batch_destroy(h)
     return (out, chunks, nnzs) if with_nnz else (out, chunks)
+
+
+# ---------------------------------------------------------------------------
+# device plan: the analysis / fill / numeric stages one product at a time
+# ---------------------------------------------------------------------------
+SELECT_HEURISTIC, SELECT_LEARNED = 0, 1
+SELECTOR_SIZER, SELECTOR_CHOOSER = 0, 1
+NB_HASH0, NB_DENSE0, NB_WIN, NB_COUNT = 1, 29, 53, 54
+SB_EMPTY, SB_H64, SB_BMS, SB_BIG = 0, 1, 8, 9
+
+
+class Plan:
+    """rvsp_hip_plan_t over device operands (kept referenced), with C's
+    structure and values in device buffers once filled."""
+
+    def __init__(self, dA: DeviceCsr, dB: DeviceCsr, select=SELECT_HEURISTIC):
+        h = C.c_void_p()
+        check("plan_create", lib().rvsp_hip_plan_create(C.byref(h), None))
+        self.h = h.value
+        self.dA, self.dB = dA, dB
+        self.analyse(select)
+
+    def analyse(self, select=SELECT_HEURISTIC):
+        nnz, prod = C.c_int64(0), C.c_int64(0)
+        dA, dB = self.dA, self.dB
+        check("plan_analyse", lib().rvsp_hip_plan_analyse(
+            self.h, dA.rows, dA.cols, dB.cols, dA.nnz, dA.rp.ptr, dA.ci.ptr, dB.nnz, dB.rp.ptr,
+            dB.ci.ptr, select, C.byref(nnz), C.byref(prod)))
+        self.nnz, self.products = nnz.value, prod.value
+        self.cci = DeviceBuffer(4 * max(self.nnz, 1))
+        self.cval = DeviceBuffer(4 * max(self.nnz, 1))
+        self.filled = False
+
+    def fill(self):
+        check("plan_fill", lib().rvsp_hip_plan_fill(self.h, self.cci.ptr))
+        check("plan_check", lib().rvsp_hip_plan_check(self.h))
+        self.filled = True
+
+    def numeric(self, sync=True):
+        if not self.filled:
+            self.fill()
+        check("plan_numeric", lib().rvsp_hip_plan_numeric(
+            self.h, self.dA.va.ptr, self.dB.va.ptr, self.cci.ptr, self.cval.ptr))
+        if sync:
+            check("plan_check", lib().rvsp_hip_plan_check(self.h))
+
+    def stream(self):
+        return lib().rvsp_hip_plan_stream(self.h)
+
+    def row_ptr(self) -> np.ndarray:
+        rp = np.empty(self.dA.rows + 1, np.int32)
+        check("d2h", lib().rvsp_hip_memcpy_d2h(rp.ctypes.data, lib().rvsp_hip_plan_row_ptr(self.h),
+                                                rp.nbytes))
+        return rp
+
+    def result(self) -> Csr:
+        return Csr(self.dA.rows, self.dB.cols, self.row_ptr(),
+                   self.cci.to_array(np.int32, self.nnz), self.cval.to_array(np.float32, self.nnz))
+
+    def bins(self):
+        m = self.dA.rows
+        sb, nb = np.empty(m, np.uint8), np.empty(m, np.uint8)
+        check("plan_get_bins", lib().rvsp_hip_plan_get_bins(self.h, sb.ctypes.data, nb.ctypes.data))
+        return sb, nb
+
+    def set_num_bins(self, nb: np.ndarray) -> int:
+        nb = np.ascontiguousarray(nb, np.uint8)
+        return lib().rvsp_hip_plan_set_num_bins(self.h, nb.ctypes.data)
+
+    def features(self):
+        """(device feats m x 5, device sizer target m) as DeviceBuffers."""
+        m = self.dA.rows
+        f, r = DeviceBuffer(20 * m), DeviceBuffer(4 * m)
+        check("plan_features", lib().rvsp_hip_plan_features(self.h, f.ptr, r.ptr))
+        return f, r
+
+    def selector_stats(self):
+        a, b = C.c_int64(0), C.c_int64(0)
+        check("selector_stats", lib().rvsp_hip_plan_selector_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def time_bin(self, b: int, rows_dev: DeviceBuffer, nrows: int, reps=3) -> float:
+        if not self.filled:
+            self.fill()
+        ms = C.c_float(0)
+        st = lib().rvsp_hip_plan_time_bin(self.h, int(b), rows_dev.ptr, int(nrows),
+                                          self.dA.va.ptr, self.dB.va.ptr, self.cci.ptr,
+                                          self.cval.ptr, int(reps), C.byref(ms))
+        check("plan_time_bin", st)
+        return ms.value
+
+    def close(self):
+        if self.h:
+            lib().rvsp_hip_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def selector_param_count(which: int) -> int:
+    return lib().rvsp_selector_param_count(which)
+
+
+def selector_set(which: int, params: np.ndarray, margin: float = 1.0):
+    p = np.ascontiguousarray(params, np.float32)
+    check("selector_set", lib().rvsp_selector_set(which, p.ctypes.data, p.size, margin))
+
+
+def selector_get(which: int):
+    n = selector_param_count(which)
+    p = np.empty(n, np.float32)
+    mg = C.c_float(0)
+    st = lib().rvsp_selector_get(which, p.ctypes.data, n, C.byref(mg))
+    return (p, mg.value) if st == SUCCESS else None
+
+
+def selector_clear(which: int):
+    check("selector_clear", lib().rvsp_selector_clear(which))
+
+
+def selector_load(which: int, path: str):
+    check("selector_load", lib().rvsp_selector_load_file(which, path.encode()))
+
+
+def selector_predict(which: int, feats_dev: DeviceBuffer, n: int, params: np.ndarray, nout: int):
+    p = np.ascontiguousarray(params, np.float32)
+    out = DeviceBuffer(4 * max(n, 1) * nout)
+    check("selector_predict", lib().rvsp_selector_predict(which, feats_dev.ptr, n, p.ctypes.data,
+                                                          out.ptr))
+    return out.to_array(np.float32, n * nout).reshape(n, nout)
+
+
+def selector_grad(which: int, feats_dev: DeviceBuffer, target_dev: DeviceBuffer, n: int,
+                  params: np.ndarray):
+    p = np.ascontiguousarray(params, np.float32)
+    g = np.empty(p.size, np.float32)
+    loss = C.c_float(0)
+    check("selector_grad", lib().rvsp_selector_grad(which, feats_dev.ptr, target_dev.ptr, n,
+                                                    p.ctypes.data, g.ctypes.data, C.byref(loss)))
+    return g, loss.value
+
+
+def selector_train(which: int, feats_dev: DeviceBuffer, target_dev: DeviceBuffer, n: int,
+                   params: np.ndarray, epochs=20, batch=4096, lr=3e-3, seed=1):
+    p = np.array(params, np.float32)
+    loss = C.c_float(0)
+    check("selector_train", lib().rvsp_selector_train(which, feats_dev.ptr, target_dev.ptr, n,
+                                                      epochs, batch, lr, seed, p.ctypes.data,
+                                                      C.byref(loss)))
+    return p, loss.value
 
 
 class Event:

@@ -10,13 +10,14 @@ namespace rvsp {
 // analysis phase.
 struct AnalysisCounters {
     int sym_hist[16];
+    int fb_hist[16];               // learned sizing: rows recounted, by heuristic bin
     int num_hist[64];
     unsigned long long products;
     unsigned long long bits_cap;   // sum over big rows of bitmap words
     unsigned long long bits_fill;  // running allocation of bitmap words
     unsigned long long cur_total;  // cursor ints over all big-row segments
     int nseg;                      // big-row segments
-    int pad;
+    int moved;                     // learned sizing: rows in a smaller table
     long long nnz_total;           // scan total
 };
 
@@ -64,23 +65,41 @@ void launch_row_stats(hipStream_t st, int32_t m, const int32_t *arp, const int32
                       int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr);
 void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int32_t *off,
                         int32_t *cursor, int32_t *perm, int nb);
+// learned: hash bins use the overflow-checking count (a row whose count
+// exceeds H/2 reports H + 1 instead of raising an error).
 void launch_sym_count(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
-                      const int32_t *off);
+                      const int32_t *off, bool learned);
 void launch_sym_fill(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
                      const int32_t *off);
 void launch_scan(hipStream_t st, int32_t m, const int32_t *cnt, int64_t *tsum, int64_t *total,
                  int32_t *crp);
 int scan_tiles(int32_t m);
+// chooser: device parameters of the learned accumulator chooser, or null
+// for the heuristic.
 void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_t *ops,
                      const int32_t *arp, const uint8_t *sbin, const int32_t *lo,
-                     const int32_t *hi, uint8_t *nbin, AnalysisCounters *ctr);
+                     const int32_t *hi, const float *chooser, uint8_t *nbin,
+                     AnalysisCounters *ctr);
 void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
                     const int32_t *off, const BigArgs &big, int nsegs);
+// One hash or dense numeric bin over the given rows (false: not such a bin).
+bool launch_num_bin(hipStream_t st, const NumArgs &a, int bin, const int32_t *rows, int nrows);
 void launch_sym_big(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g);
 void launch_big_fill(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,
                      const int32_t *crp, int32_t *cci);
 void launch_big_plan(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,
                      const int32_t *crp, AnalysisCounters *ctr);
 void launch_num_big(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g);
+
+// learned selectors (rvsp_selector.hip)
+const float *selector_device_params(int which, float *margin);
+void launch_sel_bins(hipStream_t st, int32_t m, const float *params, float margin,
+                     const int64_t *ops, const int32_t *arp, const int32_t *lo, const int32_t *hi,
+                     uint8_t *sbin, uint8_t *hbin, AnalysisCounters *ctr);
+void launch_sel_fallback(hipStream_t st, int32_t m, const int32_t *cnt, uint8_t *sbin,
+                         const uint8_t *hbin, uint8_t *fbin, AnalysisCounters *ctr);
+void launch_sel_features(hipStream_t st, int32_t m, const int64_t *ops, const int32_t *arp,
+                         const int32_t *lo, const int32_t *hi, const int32_t *crp, float *feats,
+                         float *ratio);
 
 } // namespace rvsp

@@ -161,67 +161,69 @@ static void launch_dense(hipStream_t st, const NumArgs &a, const int32_t *rows, 
 #define RVSP_HASH_CASE(HI, GI, H, G)                                                            \
     case NB_HASH0 + 4 * (HI) + (GI):                                                            \
         launch_hash<G, H>(st, a, rows, nr);                                                     \
-        break;
+        return true;
 #define RVSP_DENSE_CASE(WI, GI, W, G)                                                           \
     case NB_DENSE0 + 4 * (WI) + (GI):                                                           \
         launch_dense<G, W>(st, a, rows, nr);                                                    \
-        break;
+        return true;
+
+bool launch_num_bin(hipStream_t st, const NumArgs &a, int bin, const int32_t *rows, int nr)
+{
+    if (nr <= 0) return true;
+    switch (bin) {
+        RVSP_HASH_CASE(0, 0, 64, 8)
+        RVSP_HASH_CASE(0, 1, 64, 16)
+        RVSP_HASH_CASE(0, 2, 64, 32)
+        RVSP_HASH_CASE(0, 3, 64, 64)
+        RVSP_HASH_CASE(1, 0, 128, 8)
+        RVSP_HASH_CASE(1, 1, 128, 16)
+        RVSP_HASH_CASE(1, 2, 128, 32)
+        RVSP_HASH_CASE(1, 3, 128, 64)
+        RVSP_HASH_CASE(2, 0, 256, 8)
+        RVSP_HASH_CASE(2, 1, 256, 16)
+        RVSP_HASH_CASE(2, 2, 256, 32)
+        RVSP_HASH_CASE(2, 3, 256, 64)
+        RVSP_HASH_CASE(3, 0, 512, 8)
+        RVSP_HASH_CASE(3, 1, 512, 16)
+        RVSP_HASH_CASE(3, 2, 512, 32)
+        RVSP_HASH_CASE(3, 3, 512, 64)
+        RVSP_HASH_CASE(4, 0, 1024, 8)
+        RVSP_HASH_CASE(4, 1, 1024, 16)
+        RVSP_HASH_CASE(4, 2, 1024, 32)
+        RVSP_HASH_CASE(4, 3, 1024, 64)
+        RVSP_HASH_CASE(5, 1, 2048, 16)
+        RVSP_HASH_CASE(5, 2, 2048, 32)
+        RVSP_HASH_CASE(5, 3, 2048, 64)
+        RVSP_HASH_CASE(6, 2, 4096, 32)
+        RVSP_HASH_CASE(6, 3, 4096, 64)
+        RVSP_DENSE_CASE(0, 0, 512, 8)
+        RVSP_DENSE_CASE(0, 1, 512, 16)
+        RVSP_DENSE_CASE(0, 2, 512, 32)
+        RVSP_DENSE_CASE(0, 3, 512, 64)
+        RVSP_DENSE_CASE(1, 0, 1024, 8)
+        RVSP_DENSE_CASE(1, 1, 1024, 16)
+        RVSP_DENSE_CASE(1, 2, 1024, 32)
+        RVSP_DENSE_CASE(1, 3, 1024, 64)
+        RVSP_DENSE_CASE(2, 0, 2048, 8)
+        RVSP_DENSE_CASE(2, 1, 2048, 16)
+        RVSP_DENSE_CASE(2, 2, 2048, 32)
+        RVSP_DENSE_CASE(2, 3, 2048, 64)
+        RVSP_DENSE_CASE(3, 1, 4096, 16)
+        RVSP_DENSE_CASE(3, 2, 4096, 32)
+        RVSP_DENSE_CASE(3, 3, 4096, 64)
+        RVSP_DENSE_CASE(4, 2, 8192, 32)
+        RVSP_DENSE_CASE(4, 3, 8192, 64)
+        RVSP_DENSE_CASE(5, 3, 16384, 64)
+    default:
+        return false;
+    }
+}
 
 void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
                     const int32_t *off, const BigArgs &big, int nsegs)
 {
-    for (int b = NB_HASH0; b < NB_COUNT; ++b) {
-        const int nr = count[b];
-        if (nr == 0) continue;
-        const int32_t *rows = perm + off[b];
-        switch (b) {
-            RVSP_HASH_CASE(0, 0, 64, 8)
-            RVSP_HASH_CASE(0, 1, 64, 16)
-            RVSP_HASH_CASE(0, 2, 64, 32)
-            RVSP_HASH_CASE(0, 3, 64, 64)
-            RVSP_HASH_CASE(1, 0, 128, 8)
-            RVSP_HASH_CASE(1, 1, 128, 16)
-            RVSP_HASH_CASE(1, 2, 128, 32)
-            RVSP_HASH_CASE(1, 3, 128, 64)
-            RVSP_HASH_CASE(2, 0, 256, 8)
-            RVSP_HASH_CASE(2, 1, 256, 16)
-            RVSP_HASH_CASE(2, 2, 256, 32)
-            RVSP_HASH_CASE(2, 3, 256, 64)
-            RVSP_HASH_CASE(3, 0, 512, 8)
-            RVSP_HASH_CASE(3, 1, 512, 16)
-            RVSP_HASH_CASE(3, 2, 512, 32)
-            RVSP_HASH_CASE(3, 3, 512, 64)
-            RVSP_HASH_CASE(4, 0, 1024, 8)
-            RVSP_HASH_CASE(4, 1, 1024, 16)
-            RVSP_HASH_CASE(4, 2, 1024, 32)
-            RVSP_HASH_CASE(4, 3, 1024, 64)
-            RVSP_HASH_CASE(5, 1, 2048, 16)
-            RVSP_HASH_CASE(5, 2, 2048, 32)
-            RVSP_HASH_CASE(5, 3, 2048, 64)
-            RVSP_HASH_CASE(6, 2, 4096, 32)
-            RVSP_HASH_CASE(6, 3, 4096, 64)
-            RVSP_DENSE_CASE(0, 0, 512, 8)
-            RVSP_DENSE_CASE(0, 1, 512, 16)
-            RVSP_DENSE_CASE(0, 2, 512, 32)
-            RVSP_DENSE_CASE(0, 3, 512, 64)
-            RVSP_DENSE_CASE(1, 0, 1024, 8)
-            RVSP_DENSE_CASE(1, 1, 1024, 16)
-            RVSP_DENSE_CASE(1, 2, 1024, 32)
-            RVSP_DENSE_CASE(1, 3, 1024, 64)
-            RVSP_DENSE_CASE(2, 0, 2048, 8)
-            RVSP_DENSE_CASE(2, 1, 2048, 16)
-            RVSP_DENSE_CASE(2, 2, 2048, 32)
-            RVSP_DENSE_CASE(2, 3, 2048, 64)
-            RVSP_DENSE_CASE(3, 1, 4096, 16)
-            RVSP_DENSE_CASE(3, 2, 4096, 32)
-            RVSP_DENSE_CASE(3, 3, 4096, 64)
-            RVSP_DENSE_CASE(4, 2, 8192, 32)
-            RVSP_DENSE_CASE(4, 3, 8192, 64)
-            RVSP_DENSE_CASE(5, 3, 16384, 64)
-        default:
-            break;
-        }
-    }
+    for (int b = NB_HASH0; b < NB_WIN; ++b)
+        if (count[b] > 0) (void)launch_num_bin(st, a, b, perm + off[b], count[b]);
     launch_num_big(st, nsegs, a, big);
 }
 

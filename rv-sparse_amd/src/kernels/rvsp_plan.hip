@@ -130,6 +130,7 @@ struct rvsp_hip_plan {
     int64_t *ops = nullptr;
     int32_t *lo = nullptr, *hi = nullptr;
     uint8_t *sbin = nullptr, *nbin = nullptr;
+    uint8_t *hbin = nullptr, *fbin = nullptr;  // learned sizing: heuristic bins, recount list
     int32_t *cnt = nullptr, *crp = nullptr, *sperm = nullptr, *nperm = nullptr;
     int64_t *tsum = nullptr;
     AnalysisCounters *ctr = nullptr;
@@ -145,6 +146,10 @@ struct rvsp_hip_plan {
     int32_t sym_count[SB_COUNT], sym_off[SB_COUNT];
     int32_t num_count[NB_COUNT], num_off[NB_COUNT];
     int64_t nnzc = 0, products = 0;
+    int64_t sized_rows = 0, fallback_rows = 0;  // learned sizing: rows moved, rows recounted
+    // host sources of the bin-offset uploads (symbolic, recount, numeric);
+    // each stays untouched until the stream has passed its copy
+    int32_t h_off[3][64];
     bool analysed = false;
 
     hipGraph_t graph = nullptr;
@@ -213,7 +218,6 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
                                     int32_t nnz_b, const int32_t *brp, const int32_t *bci,
                                     rvsp_select_t select, int64_t *c_nnz, int64_t *products)
 {
-    (void)select;
     if (!p || !arp || !aci || !brp || !bci) return RVSP_ERROR_NULL_POINTER;
     if (m <= 0 || k <= 0 || n <= 0 || nnz_a < 0 || nnz_b < 0) return RVSP_ERROR_INVALID_ARGUMENT;
     hipStream_t st = p->stream;
@@ -237,6 +241,8 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     const size_t o_hi = off; off = align256(off + M * 4);
     const size_t o_sbin = off; off = align256(off + M);
     const size_t o_nbin = off; off = align256(off + M);
+    const size_t o_hbin = off; off = align256(off + M);
+    const size_t o_fbin = off; off = align256(off + M);
     const size_t o_cnt = off; off = align256(off + M * 4);
     const size_t o_crp = off; off = align256(off + (M + 1) * 4);
     const size_t o_sperm = off; off = align256(off + M * 4);
@@ -253,6 +259,8 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->hi = (int32_t *)(base + o_hi);
     p->sbin = (uint8_t *)(base + o_sbin);
     p->nbin = (uint8_t *)(base + o_nbin);
+    p->hbin = (uint8_t *)(base + o_hbin);
+    p->fbin = (uint8_t *)(base + o_fbin);
     p->cnt = (int32_t *)(base + o_cnt);
     p->crp = (int32_t *)(base + o_crp);
     p->sperm = (int32_t *)(base + o_sperm);
@@ -267,15 +275,29 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     HIP_TRY(hipMemsetAsync(p->binoff, 0, 128 * 4, st));
     HIP_TRY(hipMemsetAsync(p->err, 0, 4, st));
 
+    // learned selectors (null when none is loaded: heuristic rules)
+    float margin = 1.0f, unused = 1.0f;
+    const float *sizer =
+        select == RVSP_SELECT_LEARNED ? selector_device_params(RVSP_SELECTOR_SIZER, &margin) : nullptr;
+    const float *chooser =
+        select == RVSP_SELECT_LEARNED ? selector_device_params(RVSP_SELECTOR_CHOOSER, &unused) : nullptr;
+    p->sized_rows = p->fallback_rows = 0;
+
     // ---- 1. per-row product counts, spans and symbolic bins
     launch_row_stats(st, m, arp, aci, brp, bci, p->ops, p->lo, p->hi, p->sbin, p->ctr);
     HIP_TRY(hipGetLastError());
+    if (sizer) {
+        launch_sel_bins(st, m, sizer, margin, p->ops, arp, p->lo, p->hi, p->sbin, p->hbin, p->ctr);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipMemcpyAsync(&p->hctr, p->ctr, sizeof(AnalysisCounters), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     p->products = (int64_t)p->hctr.products;
 
-    int32_t hoff[64] = {0};
-    {
+    int32_t *hoff = p->h_off[0];
+    memset(p->h_off, 0, sizeof(p->h_off));
+    // bin lists of the symbolic pass from the counters' histogram
+    auto scatter_sym = [&]() -> rvsp_status_t {
         int run = 0;
         for (int b = 0; b < SB_COUNT; ++b) {
             p->sym_count[b] = p->hctr.sym_hist[b];
@@ -283,13 +305,18 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
             hoff[b] = run;
             run += p->sym_count[b];
         }
-    }
-    HIP_TRY(hipMemcpyAsync(p->binoff, hoff, 64 * 4, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemsetAsync(p->binoff + 64, 0, 64 * 4, st));
-    launch_bin_scatter(st, m, p->sbin, p->binoff, p->binoff + 64, p->sperm, SB_COUNT);
-    HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(p->binoff, hoff, 64 * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemsetAsync(p->binoff + 64, 0, 64 * 4, st));
+        launch_bin_scatter(st, m, p->sbin, p->binoff, p->binoff + 64, p->sperm, SB_COUNT);
+        HIP_TRY(hipGetLastError());
+        return RVSP_SUCCESS;
+    };
+    s = scatter_sym();
+    if (s != RVSP_SUCCESS) return s;
 
-    const int nbig = p->sym_count[SB_BIG];
+    // bitmaps for every row the heuristic sends to the big-row path (a
+    // superset of the learned big rows: overflowing rows may return there)
+    const bool any_big = p->hctr.bits_cap > 0;
     p->big = BigArgs{};
     p->big.arp = arp;
     p->big.aci = aci;
@@ -300,7 +327,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->big.ops = p->ops;
     p->big.cnt = p->cnt;
     p->big.bits_fill = &p->ctr->bits_fill;
-    if (nbig > 0) {
+    if (any_big) {
         const size_t words = (size_t)p->hctr.bits_cap;
         const size_t chunks = words / kChunkWords;
         size_t bo = 0;
@@ -332,17 +359,51 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     SymArgs sa{arp, aci, brp, bci, p->lo, p->hi, p->cnt, p->crp, nullptr, p->big, p->err};
     // rows without products get no kernel, so their count must start at 0
     HIP_TRY(hipMemsetAsync(p->cnt, 0, M * 4, st));
-    launch_sym_count(st, sa, p->sperm, p->sym_count, p->sym_off);
+    const bool sized = sizer && p->hctr.moved > 0;
+    launch_sym_count(st, sa, p->sperm, p->sym_count, p->sym_off, sized);
     HIP_TRY(hipGetLastError());
+    if (sized) {
+        // rows whose learned table overflowed go back to their heuristic
+        // bin and are counted again there; the bin lists are rebuilt
+        launch_sel_fallback(st, m, p->cnt, p->sbin, p->hbin, p->fbin, p->ctr);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(&p->hctr, p->ctr, sizeof(AnalysisCounters), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        int nfb = 0;
+        for (int b = 0; b < SB_COUNT; ++b) nfb += p->hctr.fb_hist[b];
+        p->sized_rows = p->hctr.moved;
+        p->fallback_rows = nfb;
+        if (nfb > 0) {
+            int32_t fcount[SB_COUNT], foff[SB_COUNT];
+            int32_t *fho = p->h_off[1];
+            int run = 0;
+            for (int b = 0; b < SB_COUNT; ++b) {
+                fcount[b] = b == SB_EMPTY ? m - nfb : p->hctr.fb_hist[b];
+                foff[b] = run;
+                fho[b] = run;
+                run += fcount[b];
+            }
+            HIP_TRY(hipMemcpyAsync(p->binoff, fho, 64 * 4, hipMemcpyHostToDevice, st));
+            HIP_TRY(hipMemsetAsync(p->binoff + 64, 0, 64 * 4, st));
+            // nperm is free until the numeric bins are scattered
+            launch_bin_scatter(st, m, p->fbin, p->binoff, p->binoff + 64, p->nperm, SB_COUNT);
+            HIP_TRY(hipGetLastError());
+            launch_sym_count(st, sa, p->nperm, fcount, foff, false);
+            HIP_TRY(hipGetLastError());
+            s = scatter_sym();
+            if (s != RVSP_SUCCESS) return s;
+        }
+    }
     // ---- 3. row pointer
     launch_scan(st, m, p->cnt, p->tsum, (int64_t *)&p->ctr->nnz_total, p->crp);
     HIP_TRY(hipGetLastError());
     // ---- 4. big-row windows and segments, numeric bins
-    if (nbig > 0) {
-        launch_big_plan(st, p->sperm + p->sym_off[SB_BIG], nbig, p->big, p->crp, p->ctr);
+    if (p->sym_count[SB_BIG] > 0) {
+        launch_big_plan(st, p->sperm + p->sym_off[SB_BIG], p->sym_count[SB_BIG], p->big, p->crp,
+                        p->ctr);
         HIP_TRY(hipGetLastError());
     }
-    launch_num_bins(st, m, p->crp, p->ops, arp, p->sbin, p->lo, p->hi, p->nbin, p->ctr);
+    launch_num_bins(st, m, p->crp, p->ops, arp, p->sbin, p->lo, p->hi, chooser, p->nbin, p->ctr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&p->hctr, p->ctr, sizeof(AnalysisCounters), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -356,9 +417,9 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
         }
         p->big.cursors = p->cursors;
     }
+    hoff = p->h_off[2];
     {
         int run = 0;
-        memset(hoff, 0, sizeof(hoff));
         for (int b = 0; b < NB_COUNT; ++b) {
             p->num_count[b] = p->hctr.num_hist[b];
             p->num_off[b] = run;
@@ -450,6 +511,110 @@ rvsp_status_t rvsp_hip_plan_check(rvsp_hip_plan_t *p)
     int herr = 0;
     HIP_TRY(hipMemcpy(&herr, p->err, 4, hipMemcpyDeviceToHost));
     return herr ? RVSP_ERROR_INVALID_CSR : RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_plan_get_bins(rvsp_hip_plan_t *p, uint8_t *sym_bins, uint8_t *num_bins)
+{
+    if (!p) return RVSP_ERROR_NULL_POINTER;
+    if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
+    const size_t M = (size_t)p->m;
+    if (sym_bins) HIP_TRY(hipMemcpyAsync(sym_bins, p->sbin, M, hipMemcpyDeviceToHost, p->stream));
+    if (num_bins) HIP_TRY(hipMemcpyAsync(num_bins, p->nbin, M, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_plan_set_num_bins(rvsp_hip_plan_t *p, const uint8_t *num_bins)
+{
+    if (!p || !num_bins) return RVSP_ERROR_NULL_POINTER;
+    if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
+    hipStream_t st = p->stream;
+    std::vector<uint8_t> sb((size_t)p->m), nb((size_t)p->m);
+    HIP_TRY(hipMemcpyAsync(sb.data(), p->sbin, sb.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(nb.data(), p->nbin, nb.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    int32_t hist[64] = {0};
+    for (int32_t i = 0; i < p->m; ++i) {
+        const int b = num_bins[i];
+        // empty rows and big rows keep their bins; the rest take a hash or dense bin
+        const bool fixed = nb[i] == NB_EMPTY || nb[i] == NB_WIN;
+        if (fixed ? b != nb[i] : (b < NB_HASH0 || b >= NB_WIN)) return RVSP_ERROR_INVALID_ARGUMENT;
+        ++hist[b];
+    }
+    int32_t hoff[64] = {0};
+    int run = 0;
+    for (int b = 0; b < NB_COUNT; ++b) {
+        p->num_count[b] = hist[b];
+        p->num_off[b] = run;
+        hoff[b] = run;
+        run += hist[b];
+    }
+    HIP_TRY(hipMemcpyAsync(p->nbin, num_bins, (size_t)p->m, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p->binoff, hoff, 64 * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(p->binoff + 64, 0, 64 * 4, st));
+    launch_bin_scatter(st, p->m, p->nbin, p->binoff, p->binoff + 64, p->nperm, NB_COUNT);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->gexec = nullptr;
+    p->graph = nullptr;
+    for (auto &k : p->gkey) k = nullptr;
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_plan_features(rvsp_hip_plan_t *p, float *feats, float *ratio)
+{
+    if (!p || !feats || !ratio) return RVSP_ERROR_NULL_POINTER;
+    if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
+    launch_sel_features(p->stream, p->m, p->ops, p->arp, p->lo, p->hi, p->crp, feats, ratio);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_plan_selector_stats(const rvsp_hip_plan_t *p, int64_t *sized_rows,
+                                           int64_t *fallback_rows)
+{
+    if (!p) return RVSP_ERROR_NULL_POINTER;
+    if (sized_rows) *sized_rows = p->sized_rows;
+    if (fallback_rows) *fallback_rows = p->fallback_rows;
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_plan_time_bin(rvsp_hip_plan_t *p, int32_t bin, const int32_t *rows,
+                                     int32_t nrows, const float *aval, const float *bval,
+                                     const int32_t *cci, float *cval, int32_t reps, float *ms)
+{
+    if (!p || !rows || !aval || !bval || !cci || !cval || !ms) return RVSP_ERROR_NULL_POINTER;
+    if (!p->analysed || nrows <= 0 || reps <= 0 || bin < NB_HASH0 || bin >= NB_WIN)
+        return RVSP_ERROR_INVALID_ARGUMENT;
+    hipStream_t st = p->stream;
+    const NumArgs a = num_args(p, aval, bval, cci, cval);
+    HIP_TRY(hipMemsetAsync(p->err, 0, 4, st));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return RVSP_ERROR_UNSUPPORTED_BACKEND;
+    }
+    rvsp_status_t s = RVSP_SUCCESS;
+    (void)launch_num_bin(st, a, bin, rows, nrows);  // warm-up
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps; ++r) (void)launch_num_bin(st, a, bin, rows, nrows);
+    (void)hipEventRecord(e1, st);
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+        s = RVSP_ERROR_UNSUPPORTED_BACKEND;
+    else
+        *ms /= (float)reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (s != RVSP_SUCCESS) return s;
+    int herr = 0;
+    HIP_TRY(hipMemcpy(&herr, p->err, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemsetAsync(p->err, 0, 4, st));
+    return herr ? RVSP_ERROR_INVALID_ARGUMENT : RVSP_SUCCESS;
 }
 
 rvsp_status_t rvsp_hip_stream_sync(void *stream)

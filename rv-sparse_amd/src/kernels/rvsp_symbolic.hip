@@ -11,6 +11,7 @@
 // same (ascending) order.
 #include "rvsp_device.hpp"
 #include "rvsp_kernels.hpp"
+#include "rvsp_mlp.hpp"
 
 namespace rvsp {
 
@@ -193,6 +194,78 @@ __global__ __launch_bounds__(256) void k_sym_hash(const int32_t *__restrict__ ro
 }
 
 // ---------------------------------------------------------------------------
+// Count for rows whose table was sized by the learned sizer: the exact
+// count when it fits the fill pass (<= H/2), else H + 1. A row stops
+// early once its count passes H/2 (checked per A entry) or its table is
+// full, so a badly underestimated row costs about H inserts.
+// ---------------------------------------------------------------------------
+template <int H>
+__device__ __forceinline__ int hash_try_insert(int32_t *T, int32_t col)
+{
+    constexpr int LG = ilog2(H);
+    uint32_t s = hash_slot(col, LG);
+    for (int probe = 0; probe < H; ++probe) {
+        int32_t cur = __hip_atomic_load(&T[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == col) return 0;
+        if (cur == -1) {
+            int32_t old = atomicCAS(&T[s], -1, col);
+            if (old == -1) return 1;
+            if (old == col) return 0;
+        }
+        s = (s + 1) & (H - 1);
+    }
+    return -1;  // full
+}
+
+template <int G, int H>
+__global__ __launch_bounds__(256) void k_sym_hash_ovf(const int32_t *__restrict__ rows,
+                                                      int32_t nrows,
+                                                      const int32_t *__restrict__ arp,
+                                                      const int32_t *__restrict__ aci,
+                                                      const int32_t *__restrict__ brp,
+                                                      const int32_t *__restrict__ bci,
+                                                      int32_t *__restrict__ cnt)
+{
+    constexpr int R = 256 / G;
+    constexpr int P = H / 2;
+    __shared__ int32_t keys[R * H];
+    __shared__ int counts[R];
+    const int g = threadIdx.x / G, lane = threadIdx.x % G;
+    const int64_t ridx = (int64_t)blockIdx.x * R + g;
+    const bool valid = ridx < nrows;
+    const int row = valid ? rows[ridx] : 0;
+    int32_t *T = keys + g * H;
+    for (int s = lane; s < H; s += G) T[s] = -1;
+    if (lane == 0) counts[g] = 0;
+    __syncthreads();
+    if (valid) {
+        const int as = arp[row], ae = arp[row + 1];
+        bool stop = false;
+        for (int ap = as; ap < ae && !stop; ++ap) {
+            if (__hip_atomic_load(&counts[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > P)
+                break;
+            const int k = aci[ap];
+            const int bs = brp[k], be = brp[k + 1];
+            for (int bp = bs + lane; bp < be; bp += G) {
+                const int r = hash_try_insert<H>(T, bci[bp]);
+                if (r > 0) {
+                    atomicAdd(&counts[g], 1);
+                } else if (r < 0) {
+                    atomicAdd(&counts[g], H);  // full: certainly over H/2
+                    stop = true;
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (valid && lane == 0) {
+        const int c = counts[g];
+        cnt[row] = c > P ? H + 1 : c;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Symbolic pass, bitmap variant for rows whose product columns span at most
 // 32*WORDS columns: one bit per column from the row's smallest product
 // column. Fill mode emits set bits in ascending order, which is C_i's column
@@ -365,11 +438,15 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
                                                   const uint8_t *__restrict__ sbin,
                                                   const int32_t *__restrict__ lo,
                                                   const int32_t *__restrict__ hi,
+                                                  const float *__restrict__ chooser,
                                                   uint8_t *__restrict__ nbin,
                                                   AnalysisCounters *__restrict__ ctr)
 {
     __shared__ int hist[NB_COUNT];
+    __shared__ float P[Chooser::NP];
     if (threadIdx.x < NB_COUNT) hist[threadIdx.x] = 0;
+    if (chooser)
+        for (int p = threadIdx.x; p < Chooser::NP; p += 256) P[p] = chooser[p];
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < m) {
@@ -379,8 +456,12 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
             b = NB_EMPTY;
         else if (sbin[i] == SB_BIG)
             b = NB_WIN;  // segments planned by k_big_plan
-        else
-            b = num_bin_of(nz, ops[i], arp[i + 1] - arp[i], (int64_t)hi[i] - lo[i] + 1);
+        else {
+            const int alen = arp[i + 1] - arp[i];
+            const int64_t span = (int64_t)hi[i] - lo[i] + 1;
+            b = chooser ? chooser_pick(P, ops[i], alen, span, nz) : -1;
+            if (b < 0) b = num_bin_of(nz, ops[i], alen, span);
+        }
         nbin[i] = (uint8_t)b;
         atomicAdd(&hist[b], 1);
     }
@@ -401,14 +482,40 @@ static void launch_sym_hash(hipStream_t st, const SymArgs &a, const int32_t *row
                                                   a.crp, a.cci, a.err);
 }
 
+template <int G, int H>
+static void launch_sym_hash_ovf(hipStream_t st, const SymArgs &a, const int32_t *rows, int nrows)
+{
+    const int R = 256 / G;
+    const int grid = (nrows + R - 1) / R;
+    k_sym_hash_ovf<G, H><<<grid, 256, 0, st>>>(rows, nrows, a.arp, a.aci, a.brp, a.bci, a.cnt);
+}
+
+static void launch_sym_learned(hipStream_t st, const SymArgs &a, int b, const int32_t *rows, int nr)
+{
+    switch (b) {
+    case SB_H64: launch_sym_hash_ovf<8, 64>(st, a, rows, nr); break;
+    case SB_H128: launch_sym_hash_ovf<16, 128>(st, a, rows, nr); break;
+    case SB_H256: launch_sym_hash_ovf<32, 256>(st, a, rows, nr); break;
+    case SB_H512: launch_sym_hash_ovf<64, 512>(st, a, rows, nr); break;
+    case SB_H1024: launch_sym_hash_ovf<64, 1024>(st, a, rows, nr); break;
+    case SB_H2048: launch_sym_hash_ovf<128, 2048>(st, a, rows, nr); break;
+    case SB_H4096: launch_sym_hash_ovf<256, 4096>(st, a, rows, nr); break;
+    default: break;
+    }
+}
+
 template <bool FILL>
 static void launch_sym_bins(hipStream_t st, const SymArgs &a, const int32_t *perm,
-                            const int32_t *count, const int32_t *off)
+                            const int32_t *count, const int32_t *off, bool learned = false)
 {
     for (int b = SB_H64; b < SB_COUNT; ++b) {
         const int nr = count[b];
         if (nr == 0) continue;
         const int32_t *rows = perm + off[b];
+        if (!FILL && learned && b >= SB_H64 && b <= SB_H4096) {
+            launch_sym_learned(st, a, b, rows, nr);
+            continue;
+        }
         switch (b) {
         case SB_H64: launch_sym_hash<8, 64, FILL>(st, a, rows, nr); break;
         case SB_H128: launch_sym_hash<16, 128, FILL>(st, a, rows, nr); break;
@@ -450,9 +557,9 @@ void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int
 }
 
 void launch_sym_count(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
-                      const int32_t *off)
+                      const int32_t *off, bool learned)
 {
-    launch_sym_bins<false>(st, a, perm, count, off);
+    launch_sym_bins<false>(st, a, perm, count, off, learned);
 }
 
 void launch_sym_fill(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
@@ -479,10 +586,11 @@ int scan_tiles(int32_t m)
 
 void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_t *ops,
                      const int32_t *arp, const uint8_t *sbin, const int32_t *lo,
-                     const int32_t *hi, uint8_t *nbin, AnalysisCounters *ctr)
+                     const int32_t *hi, const float *chooser, uint8_t *nbin,
+                     AnalysisCounters *ctr)
 {
     const int grid = (int)(((int64_t)m + 255) / 256);
-    k_num_bins<<<grid, 256, 0, st>>>(m, crp, ops, arp, sbin, lo, hi, nbin, ctr);
+    k_num_bins<<<grid, 256, 0, st>>>(m, crp, ops, arp, sbin, lo, hi, chooser, nbin, ctr);
 }
 
 } // namespace rvsp
