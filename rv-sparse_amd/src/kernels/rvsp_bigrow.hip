@@ -6,8 +6,8 @@
 // popcount per 1024-column chunk). Everything after the count reads that
 // bitmap instead of walking products again:
 //   fill     ordered compaction of the bitmap into C's column indices;
-//   plan     the row is cut into windows of <= kBigPW outputs and at most
-//            kBigMaxChunks chunks, and windows are grouped into segments;
+//   plan     the row is cut into windows of <= g.win_outs outputs and at
+//            most g.win_chunks chunks, and windows are grouped into segments;
 //   numeric  one wave per segment. For each window the wave loads the
 //            window's bitmap words into LDS with their exclusive popcount
 //            prefix (a rank dictionary: column -> position in C_i), zeroes a
@@ -24,23 +24,35 @@
 namespace rvsp {
 
 // ---------------------------------------------------------------------------
-// Symbolic count of big rows: one block per row, LDS bitmap of 2^19
-// columns per pass. B_k's entries are sorted, so a wave's 64 consecutive
-// entries fall in runs of equal bitmap words: each run is OR-combined with
-// shuffles and written with one LDS atomic.
+// Symbolic count of big rows: one block per row, an LDS bitmap of
+// g.pass_words words (sized at launch from B's column count, at most 2^19
+// columns) per pass over the row's span. Product order does not matter for
+// the structure, so the A entries of a chunk of NT are split by B-row
+// length: a thread walks a short B row alone (four loads in flight); long
+// ones are cut into pieces of kSymPiece entries shared out over the waves,
+// each lane keeping eight loads in flight. Every entry is one LDS atomic
+// OR: lanes of one instruction that hit the same word are serialised by
+// the LDS, which costs less than combining them first with lane shuffles
+// (each shuffle is itself an LDS permute instruction).
 // ---------------------------------------------------------------------------
+constexpr int kSymLong = 48;    // B rows longer than this are shared by a wave
+constexpr int kSymPiece = 512;  // entries per wave piece (8 per lane)
+
 template <int NT>
 __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows, int32_t nrows,
                                                 BigArgs g)
 {
-    __shared__ uint32_t bits[kBigPassWords];
+    extern __shared__ uint32_t bits[];
     __shared__ long long s_off;
     __shared__ int s_part[NT / 64];
+    __shared__ int s_pfx[NT + 1], s_bs[NT], s_be[NT];
+    constexpr int NWV = NT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int row = rows[blockIdx.x];
     const int32_t rlo = g.lo[row], rhi = g.hi[row];
     const int nchunks = (int)(((int64_t)rhi - rlo + kChunkCols) / kChunkCols);
     const int nwords = nchunks * kChunkWords;
+    const int passw = g.pass_words;
     if (tid == 0) {
         s_off = (long long)atomicAdd(g.bits_fill, (unsigned long long)nwords);
         g.big_off[row] = s_off;
@@ -49,15 +61,12 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
     const int64_t off = s_off;
     const int as = g.arp[row], ae = g.arp[row + 1];
     int total = 0;
-    for (int pw0 = 0; pw0 < nwords; pw0 += kBigPassWords) {
-        const int pwn = min(kBigPassWords, nwords - pw0);
+    for (int pw0 = 0; pw0 < nwords; pw0 += passw) {
+        const int pwn = min(passw, nwords - pw0);
         const int32_t base_col = rlo + pw0 * 32;
         const int32_t pcols = pwn * 32;
         for (int w = tid; w < pwn; w += NT) bits[w] = 0u;
         __syncthreads();
-        // A entries in chunks of NT: with short B rows each thread walks its
-        // own B row (order does not matter for the structure); with long
-        // ones the block covers one B row at a time.
         for (int cb = as; cb < ae; cb += NT) {
             const int myap = cb + tid;
             int mbs = 0, mbe = 0;
@@ -66,72 +75,65 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
                 mbs = g.brp[k];
                 mbe = g.brp[k + 1];
             }
-            const int cnt = min(NT, ae - cb);
-            int len = mbe - mbs;
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) len += __shfl_xor(len, o, 64);
-            if (lane == 0) s_part[wv] = len;
+            const bool lng = mbe - mbs > kSymLong;
+            int64_t npieces;
+            const int pfx =
+                (int)block_excl_scan(lng ? (mbe - mbs + kSymPiece - 1) / kSymPiece : 0, &npieces);
+            s_pfx[tid] = pfx;
+            s_bs[tid] = mbs;
+            s_be[tid] = mbe;
+            if (tid == 0) s_pfx[NT] = (int)npieces;
             __syncthreads();
-            int tot = 0;
+            if (!lng) {  // this thread's own short B row
+                for (int bp = mbs; bp < mbe; bp += 4) {
+                    int32_t c[4];
 #pragma unroll
-            for (int i = 0; i < NT / 64; ++i) tot += s_part[i];
-            __syncthreads();
-            if (tot < 64 * cnt) {
-                for (int bp = mbs; bp < mbe; ++bp) {
-                    const int32_t c = g.bci[bp] - base_col;
-                    if (c >= 0 && c < pcols) atomicOr(&bits[c >> 5], 1u << (c & 31));
-                }
-                continue;
-            }
-            for (int ap = cb; ap < cb + cnt; ++ap) {
-                const int k = g.aci[ap];
-                const int bs = g.brp[k], be = g.brp[k + 1];
-                // 4 entries per thread in flight, then OR-combine each slot
-                for (int base = bs; base < be; base += 4 * NT) {
-                    int32_t col[4];
+                    for (int u = 0; u < 4; ++u) c[u] = bp + u < mbe ? g.bci[bp + u] - base_col : -1;
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int bp = base + u * NT + tid;
-                        col[u] = bp < be ? g.bci[bp] : -1;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        if (base + u * NT >= be) break;  // block-uniform
-                        const bool valid = base + u * NT + tid < be;
-                        const int32_t c = col[u] - base_col;
-                        const bool in = valid && c >= 0 && c < pcols;
-                        int wd = in ? (c >> 5) : -1 - lane;
-                        uint32_t wb = in ? (1u << (c & 31)) : 0u;
-#pragma unroll
-                        for (int o = 1; o < 64; o <<= 1) {
-                            const uint32_t ob = __shfl_up(wb, o, 64);
-                            const int ow = __shfl_up(wd, o, 64);
-                            if (lane >= o && ow == wd) wb |= ob;
-                        }
-                        const int nwd = __shfl_down(wd, 1, 64);
-                        const bool last = lane == 63 || nwd != wd;
-                        if (in && last) atomicOr(&bits[wd], wb);
-                    }
+                    for (int u = 0; u < 4; ++u)
+                        if (c[u] >= 0 && c[u] < pcols) atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
                 }
             }
+            for (int pc = wv; pc < (int)npieces; pc += NWV) {
+                // entry e holding piece pc: the last e with s_pfx[e] <= pc
+                int lo = 0, hi = NT - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pfx[mid] <= pc)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                const int bs = s_bs[lo] + (pc - s_pfx[lo]) * kSymPiece;
+                const int be = min(bs + kSymPiece, s_be[lo]);
+                int32_t c[kSymPiece / 64];
+#pragma unroll
+                for (int u = 0; u < kSymPiece / 64; ++u) {
+                    const int bp = bs + u * 64 + lane;
+                    c[u] = bp < be ? g.bci[bp] - base_col : -1;
+                }
+#pragma unroll
+                for (int u = 0; u < kSymPiece / 64; ++u)
+                    if (c[u] >= 0 && c[u] < pcols) atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
+            }
+            __syncthreads();
         }
-        __syncthreads();
         for (int w = tid; w < pwn; w += NT) g.gbits[off + pw0 + w] = bits[w];
         const int nch = pwn / kChunkWords;
         int part = 0;
         for (int ch = tid; ch < nch; ch += NT) {
-            int s = 0;
+            int sc = 0;
 #pragma unroll 8
-            for (int w = 0; w < kChunkWords; ++w) s += __popc(bits[ch * kChunkWords + w]);
-            g.gchunk[off / kChunkWords + pw0 / kChunkWords + ch] = s;
-            part += s;
+            for (int w = 0; w < kChunkWords; ++w) sc += __popc(bits[ch * kChunkWords + w]);
+            g.gchunk[off / kChunkWords + pw0 / kChunkWords + ch] = sc;
+            part += sc;
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
         if (lane == 0) s_part[wv] = part;
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < NT / 64; ++i) total += s_part[i];
+        for (int i = 0; i < NWV; ++i) total += s_part[i];
         __syncthreads();
     }
     if (tid == 0) g.cnt[row] = total;
@@ -178,7 +180,7 @@ __global__ __launch_bounds__(256) void k_big_fill(const int32_t *__restrict__ ro
 
 // ---------------------------------------------------------------------------
 // Window and segment plan: one thread per big row. Windows are runs of
-// whole chunks with at most kBigPW outputs and kBigMaxChunks chunks.
+// whole chunks with at most g.win_outs outputs and g.win_chunks chunks.
 // Segments are runs of windows holding about kSegProducts products
 // (estimated from the row's products per output), each processed by one
 // wave, so the widest rows spread over many waves; their cursor blocks
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
     // pass 1: windows
     for (int c = 0; c < nchunks; ++c) {
         const int cc = g.gchunk[c32 + c];
-        if (c > start && (acc + cc > kBigPW || c - start >= kBigMaxChunks)) {
+        if (c > start && (acc + cc > g.win_outs || c - start >= g.win_chunks)) {
             g.win_chunk0[c32 + nwin] = start;
             g.win_out[c32 + nwin] = out;
             ++nwin;
@@ -261,46 +263,105 @@ __device__ __forceinline__ int lane_lower_bound(const int32_t *__restrict__ a, i
     return lo;
 }
 
+// Adds one batch of a piece (lanes hold consecutive entries of B_k, so
+// distinct columns) and returns how many of its entries lie in the window.
+__device__ __forceinline__ int big_batch(float aj, int32_t col, float bv, int32_t c0, int32_t c1,
+                                         int outs, const uint32_t *W, const uint16_t *P,
+                                         float *acc, const NumArgs &a)
+{
+    const bool in = col < c1;
+    if (in) {
+        const int rel = col - c0;
+        if (rel >= 0) {
+            const int wi = rel >> 5;
+            const uint32_t word = W[wi];
+            const int pos = (int)P[wi] + __popc(word & ((1u << (rel & 31)) - 1u));
+            if (((word >> (rel & 31)) & 1u) && pos < outs)
+                atomicAdd(&acc[pos], __fmul_rn(aj, bv));
+            else
+                atomicOr(a.err, kErrProbeMiss);
+        } else {
+            atomicOr(a.err, kErrWindow);
+        }
+    }
+    return __popcll(__ballot(in));
+}
+
 // Adds the products of one A entry's piece in [c0, c1) that start at `cj`
-// (loaded values already in col/val for the first 64 entries); returns the
-// new cursor. Lanes cover consecutive entries of B_k, so columns are
-// distinct within an instruction.
+// (the first 64 entries already loaded in col/val); returns the new
+// cursor. All adds of a piece have the same k and distinct columns, so
+// their order does not matter: a piece longer than 64 entries has its next
+// kPieceBatches batches loaded at once before their adds.
+constexpr int kPieceBatches = 4;
+
 __device__ __forceinline__ int big_piece(int cj, int ej, float aj, int32_t col, float bv,
                                          int32_t c0, int32_t c1, int outs, const uint32_t *W,
                                          const uint16_t *P, float *acc, const NumArgs &a)
 {
     const int lane = threadIdx.x & 63;
-    for (;;) {
-        const bool in = col < c1;
-        if (in) {
-            const int rel = col - c0;
-            if (rel >= 0) {
-                const int wi = rel >> 5;
-                const uint32_t word = W[wi];
-                const int pos = (int)P[wi] + __popc(word & ((1u << (rel & 31)) - 1u));
-                if (((word >> (rel & 31)) & 1u) && pos < outs)
-                    atomicAdd(&acc[pos], __fmul_rn(aj, bv));
-                else
-                    atomicOr(a.err, kErrProbeMiss);
-            } else {
-                atomicOr(a.err, kErrWindow);
+    int nin = big_batch(aj, col, bv, c0, c1, outs, W, P, acc, a);
+    cj += nin;
+    while (nin == 64) {
+        int32_t cb[kPieceBatches];
+        float vb[kPieceBatches];
+#pragma unroll
+        for (int b = 0; b < kPieceBatches; ++b) {
+            const int bp = cj + b * 64 + lane;
+            cb[b] = bp < ej ? a.bci[bp] : INT32_MAX;
+            vb[b] = bp < ej ? a.bval[bp] : 0.0f;
+        }
+#pragma unroll
+        for (int b = 0; b < kPieceBatches; ++b) {
+            if (nin == 64) {  // uniform: the previous batch lay wholly in the window
+                nin = big_batch(aj, cb[b], vb[b], c0, c1, outs, W, P, acc, a);
+                cj += nin;
             }
         }
-        const int nin = __popcll(__ballot(in));
-        cj += nin;
-        if (nin < 64) return cj;
-        const int bp = cj + lane;  // piece continues past 64 entries
-        col = bp < ej ? a.bci[bp] : INT32_MAX;
-        bv = bp < ej ? a.bval[bp] : 0.0f;
     }
+    return cj;
 }
 
-template <int NW>
+// Up to four live A entries of a window, taken from a lane mask in lane
+// (= ascending k) order, with the first 64 entries of each piece loaded.
+struct PieceGroup {
+    int n;
+    int j[4], c[4], e[4];
+    float av[4], bv[4];
+    int32_t col[4];
+
+    __device__ __forceinline__ void take(uint64_t &mask, int2 ce, float a_val, const NumArgs &a)
+    {
+        const int lane = threadIdx.x & 63;
+        n = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            j[u] = 0;
+            if (mask) {
+                j[u] = __ffsll((unsigned long long)mask) - 1;
+                mask &= mask - 1;
+                n = u + 1;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c[u] = lane_bcast(ce.x, j[u]);
+            e[u] = lane_bcast(ce.y, j[u]);
+            av[u] = lane_bcast(a_val, j[u]);
+            const int bp = c[u] + lane;
+            const bool ok = u < n && bp < e[u];
+            col[u] = ok ? a.bci[bp] : INT32_MAX;
+            bv[u] = ok ? a.bval[bp] : 0.0f;
+        }
+    }
+};
+
+template <int NW, int PW, int MAXCH>
 __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, BigArgs g)
 {
-    __shared__ float acc_s[NW][kBigPW];
-    __shared__ uint32_t W_s[NW][kBigMaxWords];
-    __shared__ uint16_t P_s[NW][kBigMaxWords];
+    constexpr int MAXW = MAXCH * kChunkWords;
+    __shared__ float acc_s[NW][PW];
+    __shared__ uint32_t W_s[NW][MAXW];
+    __shared__ uint16_t P_s[NW][MAXW];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t it = (int64_t)blockIdx.x * NW + wv;
     if (it >= nsegs) return;  // whole wave leaves; no block barriers below
@@ -366,9 +427,9 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
         const int32_t c0 = rlo + ch0 * kChunkCols;
         const int64_t c1l = (int64_t)rlo + (int64_t)ch1 * kChunkCols;
         const int32_t c1 = c1l > INT32_MAX ? INT32_MAX : (int32_t)c1l;
-        const int nw = (ch1 - ch0) * kChunkWords;  // <= kBigMaxWords
+        const int nw = (ch1 - ch0) * kChunkWords;  // <= MAXW
         // rank dictionary: words and their exclusive popcount prefix
-        constexpr int PER = kBigMaxWords / 64;
+        constexpr int PER = MAXW / 64;
         int lsum = 0;
         uint32_t wl[PER];
 #pragma unroll
@@ -408,37 +469,15 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
             uint64_t mask = __ballot(live);
             while (mask) {
                 // up to four live entries: issue all their loads first
-                int js[4], cs[4], es[4];
-                float as4[4], bvs[4];
-                int32_t cols[4];
-                int nj = 0;
+                PieceGroup g0;
+                g0.take(mask, ce, av, a);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    js[u] = -1;
-                    if (mask) {
-                        js[u] = __ffsll((unsigned long long)mask) - 1;
-                        mask &= mask - 1;
-                        nj = u + 1;
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int ju = js[u] < 0 ? 0 : js[u];
-                    cs[u] = __shfl(ce.x, ju, 64);
-                    es[u] = __shfl(ce.y, ju, 64);
-                    as4[u] = __shfl(av, ju, 64);
-                    const int bp = cs[u] + lane;
-                    const bool ok = u < nj && bp < es[u];
-                    cols[u] = ok ? a.bci[bp] : INT32_MAX;
-                    bvs[u] = ok ? a.bval[bp] : 0.0f;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (u < nj) {
-                        const int nc = big_piece(cs[u], es[u], as4[u], cols[u], bvs[u], c0, c1, outs,
-                                                 W, P, acc, a);
+                    if (u < g0.n) {
+                        const int nc = big_piece(g0.c[u], g0.e[u], g0.av[u], g0.col[u], g0.bv[u],
+                                                 c0, c1, outs, W, P, acc, a);
                         compiler_fence();
-                        if (lane == js[u]) cur = nc;
+                        if (lane == g0.j[u]) cur = nc;
                     }
                 }
             }
@@ -455,7 +494,15 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
 // ---------------------------------------------------------------------------
 void launch_sym_big(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g)
 {
-    if (nrows > 0) k_sym_big<256><<<nrows, 256, 0, st>>>(rows, nrows, g);
+    if (nrows <= 0) return;
+    const size_t lds = (size_t)g.pass_words * 4;
+    static bool raised = false;  // allow the largest pass once per process
+    if (!raised) {
+        (void)hipFuncSetAttribute((const void *)&k_sym_big<256>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kBigPassWords * 4);
+        raised = true;
+    }
+    k_sym_big<256><<<nrows, 256, lds, st>>>(rows, nrows, g);
 }
 
 void launch_big_fill(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,
@@ -470,10 +517,27 @@ void launch_big_plan(hipStream_t st, const int32_t *rows, int nrows, const BigAr
     if (nrows > 0) k_big_plan<<<(nrows + 255) / 256, 256, 0, st>>>(rows, nrows, g, crp, ctr);
 }
 
-void launch_num_big(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g)
+template <int PW, int MAXCH>
+static void launch_num_big_shape(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g)
 {
     constexpr int NW = 2;
-    if (nsegs > 0) k_num_big<NW><<<(nsegs + NW - 1) / NW, NW * 64, 0, st>>>(nsegs, a, g);
+    k_num_big<NW, PW, MAXCH><<<(nsegs + NW - 1) / NW, NW * 64, 0, st>>>(nsegs, a, g);
+}
+
+void launch_num_big(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g)
+{
+    if (nsegs <= 0) return;
+    // the shape must be one of kBigShapes (the plan only sets those)
+    if (g.win_outs == 4096 && g.win_chunks == 32)
+        launch_num_big_shape<4096, 32>(st, nsegs, a, g);
+    else if (g.win_outs == 2048 && g.win_chunks == 32)
+        launch_num_big_shape<2048, 32>(st, nsegs, a, g);
+    else if (g.win_outs == 1024 && g.win_chunks == 16)
+        launch_num_big_shape<1024, 16>(st, nsegs, a, g);
+    else if (g.win_outs == 1024 && g.win_chunks == 8)
+        launch_num_big_shape<1024, 8>(st, nsegs, a, g);
+    else
+        launch_num_big_shape<2048, 16>(st, nsegs, a, g);
 }
 
 } // namespace rvsp

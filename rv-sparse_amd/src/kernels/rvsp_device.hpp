@@ -43,15 +43,20 @@ constexpr int kBmsWords = 512;            // 16384 columns
 constexpr int kBigPassWords = 16384;      // big-row bitmap: 2^19 columns per pass
 constexpr int kChunkCols = 1024;          // big-row chunk (window granularity)
 constexpr int kChunkWords = kChunkCols / 32;
-constexpr int kBigPW = 4096;              // outputs per big-row window
-constexpr int kBigMaxChunks = 32;         // chunks per big-row window
-constexpr int kBigMaxWords = kBigMaxChunks * kChunkWords;
 constexpr int kSegProducts = 65536;       // products per big-row segment (one wave)
 constexpr int kDenseMaxCols = 16384;      // widest dense window (64 KB acc)
 constexpr int kHashMaxNnz = 2730;         // numeric hash rows: H = 4096 at load <= 2/3
 constexpr int kSymHashMaxU = 2048;        // symbolic hash rows: min(ops,span) <= this
 
 __device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
+
+// Value of lane `src` (the same for every lane of the wave) broadcast to all
+// lanes through a scalar register; __shfl would use an LDS permute.
+__device__ __forceinline__ int lane_bcast(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
+__device__ __forceinline__ float lane_bcast(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
 
 // Orders LDS traffic of one wave: everything issued before is visible to
 // every lane of the wave after. Same sequence rocPRIM uses for wave-level

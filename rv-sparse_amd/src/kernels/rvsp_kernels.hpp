@@ -37,7 +37,14 @@ struct BigArgs {
     int4 *segs;           // (row, first window, end window, 0)
     int64_t *seg_cur;     // per segment: offset of its cursor block
     int2 *cursors;        // per (segment, A entry): (next B position, end of B row)
+    int pass_words;       // symbolic bitmap words per pass (LDS), multiple of kChunkWords
+    int win_outs;         // numeric window: at most this many outputs ...
+    int win_chunks;       // ... and this many chunks (one of the launchable shapes)
 };
+
+// Launchable big-row window shapes (outputs, chunks); the first is the default.
+constexpr int kBigShapes[][2] = {{2048, 16}, {4096, 32}, {2048, 32}, {1024, 16}, {1024, 8}};
+constexpr int kBigShapeCount = sizeof(kBigShapes) / sizeof(kBigShapes[0]);
 
 struct SymArgs {
     const int32_t *arp, *aci, *brp, *bci;

@@ -51,9 +51,15 @@ __device__ __forceinline__ void walk_row_ordered(const NumArgs &a, int as, int a
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int j = min(j0 + u, G - 1);
-                au[u] = __shfl(av, j, G);
-                su[u] = __shfl(bs, j, G);
-                eu[u] = __shfl(be, j, G);
+                if constexpr (G == 64) {  // one row per wave: j is wave-uniform
+                    au[u] = lane_bcast(av, j);
+                    su[u] = lane_bcast(bs, j);
+                    eu[u] = lane_bcast(be, j);
+                } else {
+                    au[u] = __shfl(av, j, G);
+                    su[u] = __shfl(bs, j, G);
+                    eu[u] = __shfl(be, j, G);
+                }
                 const int bp = su[u] + lane;
                 const bool ok = j0 + u < nch && bp < eu[u];
                 cu[u] = ok ? a.bci[bp] : -1;

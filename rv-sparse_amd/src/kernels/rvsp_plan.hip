@@ -3,6 +3,7 @@
 // and timing helpers of rv_sparse_hip.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -82,6 +83,22 @@ void dfree(void *p, hipStream_t st)
 }
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Big-row numeric window shape: kBigShapes[0] unless RVSP_BIG_WINDOW names
+// another launchable one as "<outputs>x<chunks>" (read once).
+int big_window_shape()
+{
+    static int shape = -1;
+    if (shape < 0) {
+        shape = 0;
+        const char *e = getenv("RVSP_BIG_WINDOW");
+        int o = 0, c = 0;
+        if (e && sscanf(e, "%dx%d", &o, &c) == 2)
+            for (int i = 0; i < kBigShapeCount; ++i)
+                if (kBigShapes[i][0] == o && kBigShapes[i][1] == c) shape = i;
+    }
+    return shape;
+}
 
 // Phase timing: when on, every analyse / fill / numeric call records a pair
 // of HIP events on its own stream; rvsp_hip_get_timing resolves them.
@@ -327,6 +344,15 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->big.ops = p->ops;
     p->big.cnt = p->cnt;
     p->big.bits_fill = &p->ctr->bits_fill;
+    {
+        // bitmap pass: the whole span of any row when B has few enough
+        // columns (chunk-aligned spans need at most n/1024 + 1 chunks)
+        const int64_t need = ((int64_t)n / kChunkCols + 1) * kChunkWords;
+        p->big.pass_words = (int)(need < kBigPassWords ? need : kBigPassWords);
+        const int shape = big_window_shape();
+        p->big.win_outs = kBigShapes[shape][0];
+        p->big.win_chunks = kBigShapes[shape][1];
+    }
     if (any_big) {
         const size_t words = (size_t)p->hctr.bits_cap;
         const size_t chunks = words / kChunkWords;
