@@ -140,41 +140,60 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
 }
 
 // ---------------------------------------------------------------------------
-// Structure fill of big rows from their bitmaps: one block per row, each
-// thread owns 8 consecutive words of a 2048-word tile.
+// Structure fill of big rows from their bitmaps: one block per row, tiles
+// of 1024 words (4 per thread). A tile's set bits are expanded into an LDS
+// staging buffer at their ordered positions, then copied out contiguously,
+// so the global writes are coalesced; a tile with more than kFillStage
+// columns (a nearly dense span) writes straight from the threads instead.
 // ---------------------------------------------------------------------------
+constexpr int kFillPer = 4;
+constexpr int kFillTile = 256 * kFillPer;  // words
+constexpr int kFillStage = 8192;           // staged columns per tile
+
 __global__ __launch_bounds__(256) void k_big_fill(const int32_t *__restrict__ rows, int32_t nrows,
                                                   BigArgs g, const int32_t *__restrict__ crp,
                                                   int32_t *__restrict__ cci)
 {
-    constexpr int PER = 8;
+    __shared__ int32_t stage[kFillStage];
     const int row = rows[blockIdx.x];
     const int32_t rlo = g.lo[row], rhi = g.hi[row];
     const int nwords = (int)(((int64_t)rhi - rlo + kChunkCols) / kChunkCols) * kChunkWords;
     const int64_t off = g.big_off[row];
     int out = crp[row];
-    for (int t0 = 0; t0 < nwords; t0 += 256 * PER) {
-        uint32_t w[PER];
+    for (int t0 = 0; t0 < nwords; t0 += kFillTile) {
+        uint32_t w[kFillPer];
         int local = 0;
-        const int wbase = t0 + threadIdx.x * PER;
+        const int wbase = t0 + threadIdx.x * kFillPer;
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
+        for (int j = 0; j < kFillPer; ++j) {
             w[j] = wbase + j < nwords ? g.gbits[off + wbase + j] : 0u;
             local += __popc(w[j]);
         }
-        int64_t tot;
-        int pos = out + (int)block_excl_scan(local, &tot);
+        int64_t tot64;
+        const int rel = (int)block_excl_scan(local, &tot64);
+        const int tot = (int)tot64;
+        const bool staged = tot <= kFillStage;
+        int pos = rel;
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
+        for (int j = 0; j < kFillPer; ++j) {
             uint32_t x = w[j];
             const int32_t cb = rlo + (wbase + j) * 32;
             while (x) {
                 const int b = __ffs(x) - 1;
                 x &= x - 1;
-                cci[pos++] = cb + b;
+                if (staged)
+                    stage[pos] = cb + b;
+                else
+                    cci[out + pos] = cb + b;
+                ++pos;
             }
         }
-        out += (int)tot;
+        if (staged) {  // block-uniform
+            __syncthreads();
+            for (int i = threadIdx.x; i < tot; i += 256) cci[out + i] = stage[i];
+            __syncthreads();
+        }
+        out += tot;
     }
 }
 
@@ -190,43 +209,56 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
                                                   BigArgs g, const int32_t *__restrict__ crp,
                                                   AnalysisCounters *__restrict__ ctr)
 {
+    __shared__ int s_seg;            // block's first segment index
+    __shared__ long long s_cur;      // block's first cursor entry
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (j >= nrows) return;
-    const int row = rows[j];
+    const bool valid = j < nrows;    // every thread joins the block scans below
+    const int row = valid ? rows[j] : 0;
     const int32_t rlo = g.lo[row], rhi = g.hi[row];
     const int nchunks = (int)(((int64_t)rhi - rlo + kChunkCols) / kChunkCols);
-    const int64_t c32 = g.big_off[row] / kChunkWords;
+    const int64_t c32 = valid ? g.big_off[row] / kChunkWords : 0;
     const int alen = g.arp[row + 1] - g.arp[row];
     const int nz = crp[row + 1] - crp[row];
     // products per output of this row: segments aim at ~kSegProducts each
     const double ratio = nz > 0 ? (double)g.ops[row] / (double)nz : 1.0;
-    int nwin = 0, out = 0, acc = 0, start = 0;
-    // pass 1: windows
-    for (int c = 0; c < nchunks; ++c) {
-        const int cc = g.gchunk[c32 + c];
-        if (c > start && (acc + cc > g.win_outs || c - start >= g.win_chunks)) {
-            g.win_chunk0[c32 + nwin] = start;
-            g.win_out[c32 + nwin] = out;
-            ++nwin;
-            out += acc;
-            start = c;
-            acc = 0;
+    int nwin = 0;
+    if (valid) {
+        int out = 0, acc = 0, start = 0;
+        // pass 1: windows
+        for (int c = 0; c < nchunks; ++c) {
+            const int cc = g.gchunk[c32 + c];
+            if (c > start && (acc + cc > g.win_outs || c - start >= g.win_chunks)) {
+                g.win_chunk0[c32 + nwin] = start;
+                g.win_out[c32 + nwin] = out;
+                ++nwin;
+                out += acc;
+                start = c;
+                acc = 0;
+            }
+            acc += cc;
         }
-        acc += cc;
+        g.win_chunk0[c32 + nwin] = start;
+        g.win_out[c32 + nwin] = out;
+        ++nwin;
+        g.big_nwin[row] = nwin;
     }
-    g.win_chunk0[c32 + nwin] = start;
-    g.win_out[c32 + nwin] = out;
-    ++nwin;
-    g.big_nwin[row] = nwin;
-    // pass 2: segments (count, then emit)
+    // pass 2: segments, counted, then ranges reserved for the whole block
+    // with one atomic each, then emitted
     int nseg = 0;
     for (int pass = 0; pass < 2; ++pass) {
         int sbase = 0;
         long long cbase = 0;
         if (pass == 1) {
-            sbase = atomicAdd(&ctr->nseg, nseg);
-            cbase = (long long)atomicAdd(&ctr->cur_total,
-                                         (unsigned long long)alen * (unsigned long long)nseg);
+            int64_t tseg, tcur;
+            const int64_t xseg = block_excl_scan(nseg, &tseg);
+            const int64_t xcur = block_excl_scan((int64_t)alen * nseg * (valid ? 1 : 0), &tcur);
+            if (threadIdx.x == 0) {
+                s_seg = tseg ? atomicAdd(&ctr->nseg, (int)tseg) : 0;
+                s_cur = tcur ? (long long)atomicAdd(&ctr->cur_total, (unsigned long long)tcur) : 0;
+            }
+            __syncthreads();
+            sbase = s_seg + (int)xseg;
+            cbase = s_cur + xcur;
         }
         int s = 0, w0 = 0;
         double est = 0.0;

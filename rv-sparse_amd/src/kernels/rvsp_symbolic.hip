@@ -43,7 +43,9 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
 {
     constexpr int G = 16;
     __shared__ int hist[SB_COUNT];
+    __shared__ unsigned long long bprod, bcap;  // block totals: one global atomic each
     if (threadIdx.x < SB_COUNT) hist[threadIdx.x] = 0;
+    if (threadIdx.x == 0) bprod = bcap = 0ull;
     __syncthreads();
     const int lane = threadIdx.x % G;
     const int64_t row = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
@@ -76,13 +78,17 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
         atomicAdd(&hist[b], 1);
         if (b == SB_BIG) {
             const int nchunks = (int)(((int64_t)hi - lo + kChunkCols) / kChunkCols);
-            atomicAdd(&ctr->bits_cap, (unsigned long long)nchunks * kChunkWords);
+            atomicAdd(&bcap, (unsigned long long)nchunks * kChunkWords);
         }
-        atomicAdd((unsigned long long *)&ctr->products, (unsigned long long)ops);
+        if (ops) atomicAdd(&bprod, (unsigned long long)ops);
     }
     __syncthreads();
     if (threadIdx.x < SB_COUNT && hist[threadIdx.x])
         atomicAdd(&ctr->sym_hist[threadIdx.x], hist[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        if (bprod) atomicAdd(&ctr->products, bprod);
+        if (bcap) atomicAdd(&ctr->bits_cap, bcap);
+    }
 }
 
 // ---------------------------------------------------------------------------
