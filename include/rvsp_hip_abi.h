@@ -74,6 +74,15 @@ rvsp_status_t rvsp_hip_plan_numeric_graph(rvsp_hip_plan_t *plan,
                                           const int32_t *c_col_idx,
                                           float *c_values);
 
+/* Prepares numeric passes under a reused structure: computes, once per
+ * analysis, the output rank of every product of the rows on the LDS paths
+ * (2 bytes per product, from C's column indices in c_col_idx) so later
+ * numeric passes need no column reads or probing. Skipped (the plain
+ * kernels stay in use) when it would take more than a quarter of the free
+ * device memory or when RVSP_REUSE_RANKS=0. Queued on the plan stream
+ * after one synchronisation. Drops a captured numeric graph. */
+rvsp_status_t rvsp_hip_plan_prepare_reuse(rvsp_hip_plan_t *plan, const int32_t *c_col_idx);
+
 /* Checks the device-side overflow/consistency flag the kernels raise
  * (table overflow, probe exhaustion). RVSP_ERROR_INVALID_CSR if raised;
  * synchronises the plan stream. */

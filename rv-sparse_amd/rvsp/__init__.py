@@ -143,6 +143,7 @@ def _declare(lib):
         "rvsp_hip_plan_fill": (C.c_int, [V, V]),
         "rvsp_hip_plan_numeric": (C.c_int, [V, V, V, V, V]),
         "rvsp_hip_plan_check": (C.c_int, [V]),
+        "rvsp_hip_plan_prepare_reuse": (C.c_int, [V, V]),
         "rvsp_hip_plan_get_bins": (C.c_int, [V, V, V]),
         "rvsp_hip_plan_set_num_bins": (C.c_int, [V, V]),
         "rvsp_hip_plan_features": (C.c_int, [V, V, V]),
@@ -491,6 +492,12 @@ class Plan:
             self.h, self.dA.va.ptr, self.dB.va.ptr, self.cci.ptr, self.cval.ptr))
         if sync:
             check("plan_check", lib().rvsp_hip_plan_check(self.h))
+
+    def prepare_reuse(self):
+        """Per-product output ranks for later numeric passes (once per analysis)."""
+        if not self.filled:
+            self.fill()
+        check("plan_prepare_reuse", lib().rvsp_hip_plan_prepare_reuse(self.h, self.cci.ptr))
 
     def stream(self):
         return lib().rvsp_hip_plan_stream(self.h)

@@ -311,6 +311,8 @@ rvsp_status_t rvsp_spgemm_compute(rvsp_spgemm_descr_t descr, const rvsp_csr_matr
                 return s;
             descr->filled_into = C->col_idx;
         }
+        else if ((s = rvsp_hip_plan_prepare_reuse(plan, C->col_idx)))
+            return s;
         if (descr->use_graph && !refill)
             s = rvsp_hip_plan_numeric_graph(plan, (const float *)A->values,
                                             (const float *)B->values, C->col_idx,
@@ -334,6 +336,8 @@ rvsp_status_t rvsp_spgemm_compute(rvsp_spgemm_descr_t descr, const rvsp_csr_matr
                 (s = rvsp_hip_copy_async(C->col_idx, descr->d_cci, c_bytes, st)))
                 return s;
         }
+        else if ((s = rvsp_hip_plan_prepare_reuse(plan, descr->d_cci)))
+            return s;
         if (descr->use_graph && !refill)
             s = rvsp_hip_plan_numeric_graph(plan, descr->d_aval, descr->d_bval, descr->d_cci,
                                             descr->d_cval);

@@ -43,12 +43,20 @@ constexpr int kBmsWords = 512;            // 16384 columns
 constexpr int kBigPassWords = 16384;      // big-row bitmap: 2^19 columns per pass
 constexpr int kChunkCols = 1024;          // big-row chunk (window granularity)
 constexpr int kChunkWords = kChunkCols / 32;
-constexpr int kSegProducts = 65536;       // products per big-row segment (one wave)
+constexpr int kSegProducts = 16384;       // products per big-row segment (one wave)
 constexpr int kDenseMaxCols = 16384;      // widest dense window (64 KB acc)
 constexpr int kHashMaxNnz = 2730;         // numeric hash rows: H = 4096 at load <= 2/3
 constexpr int kSymHashMaxU = 2048;        // symbolic hash rows: min(ops,span) <= this
 
 __device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
+
+// One A entry of a row walk, as the lanes of a group share it: its value,
+// its B row [bs, be), and (rank walks) the products of earlier entries of
+// its chunk. Read by every lane of the group with one 16-byte LDS load.
+struct __align__(16) KMeta {
+    float av;
+    int bs, be, pre;
+};
 
 // Value of lane `src` (the same for every lane of the wave) broadcast to all
 // lanes through a scalar register; __shfl would use an LDS permute.

@@ -98,6 +98,17 @@ void launch_big_plan(hipStream_t st, const int32_t *rows, int nrows, const BigAr
                      const int32_t *crp, AnalysisCounters *ctr);
 void launch_num_big(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g);
 
+// reused structure (rvsp_reuse.hip): per-product output ranks of the rows on
+// the LDS paths; with pmap null only the per-row product offsets are made
+void launch_pm_build(hipStream_t st, int32_t m, const uint8_t *nbin, const int64_t *ops,
+                     const int32_t *arp, const int32_t *aci, const int32_t *brp,
+                     const int32_t *bci, const int32_t *crp, const int32_t *cci, int64_t *cnt,
+                     int64_t *tsum, int64_t *total, int64_t *prow, uint16_t *pmap, int *err);
+int pm_scan_tiles(int32_t m);
+void launch_numeric_ranked(hipStream_t st, const NumArgs &a, const int32_t *perm,
+                           const int32_t *count, const int32_t *off, const BigArgs &big,
+                           int nsegs, const int64_t *prow, const uint16_t *pmap);
+
 // learned selectors (rvsp_selector.hip)
 const float *selector_device_params(int which, float *margin);
 void launch_sel_bins(hipStream_t st, int32_t m, const float *params, float margin,

@@ -28,9 +28,13 @@ namespace rvsp {
 // calls add(col, fl(a*b)) for every product. Loads for four k-steps are
 // issued before any of their adds, so up to eight loads per lane are in
 // flight; the adds of each k-step are then issued in k order, separated by
-// compiler fences (B is read-only, so hoisting its loads is safe).
+// compiler fences (B is read-only, so hoisting its loads is safe). The
+// entry of a k-step reaches the group's lanes through a scalar register
+// (G = 64: one row per wave) or through the group's LDS table M of G
+// entries (one 16-byte load per k-step instead of lane permutes).
 template <int G, typename Add>
-__device__ __forceinline__ void walk_row_ordered(const NumArgs &a, int as, int ae, Add add)
+__device__ __forceinline__ void walk_row_ordered(const NumArgs &a, int as, int ae, KMeta *M,
+                                                 Add add)
 {
     const int lane = threadIdx.x % G;
     for (int base = as; base < ae; base += G) {
@@ -42,6 +46,10 @@ __device__ __forceinline__ void walk_row_ordered(const NumArgs &a, int as, int a
             av = a.aval[ap];
             bs = a.brp[k];
             be = a.brp[k + 1];
+        }
+        if constexpr (G < 64) {
+            M[lane] = KMeta{av, bs, be, 0};
+            wave_sync();
         }
         const int nch = min(G, ae - base);
         for (int j0 = 0; j0 < nch; j0 += 4) {
@@ -56,9 +64,10 @@ __device__ __forceinline__ void walk_row_ordered(const NumArgs &a, int as, int a
                     su[u] = lane_bcast(bs, j);
                     eu[u] = lane_bcast(be, j);
                 } else {
-                    au[u] = __shfl(av, j, G);
-                    su[u] = __shfl(bs, j, G);
-                    eu[u] = __shfl(be, j, G);
+                    const KMeta e = M[j];
+                    au[u] = e.av;
+                    su[u] = e.bs;
+                    eu[u] = e.be;
                 }
                 const int bp = su[u] + lane;
                 const bool ok = j0 + u < nch && bp < eu[u];
@@ -75,6 +84,7 @@ __device__ __forceinline__ void walk_row_ordered(const NumArgs &a, int as, int a
                 }
             }
         }
+        if constexpr (G < 64) wave_sync();  // the table is rewritten for the next chunk
     }
 }
 
@@ -84,6 +94,7 @@ __global__ __launch_bounds__(R *G) void k_num_hash(const int32_t *__restrict__ r
 {
     __shared__ int32_t keys[R * H];
     __shared__ float vals[R * H];
+    __shared__ KMeta meta[G < 64 ? R * G : 1];
     const int g = threadIdx.x / G, lane = threadIdx.x % G;
     const int64_t ridx = (int64_t)blockIdx.x * R + g;
     const bool valid = ridx < nrows;
@@ -100,10 +111,11 @@ __global__ __launch_bounds__(R *G) void k_num_hash(const int32_t *__restrict__ r
     __syncthreads();
     if (valid) {
         int *err = a.err;
-        walk_row_ordered<G>(a, a.arp[row], a.arp[row + 1], [&](int32_t col, float prod) {
-            const int s = hash_find<H>(T, col, err);
-            if (s >= 0) atomicAdd(&V[s], prod);
-        });
+        walk_row_ordered<G>(a, a.arp[row], a.arp[row + 1], meta + (G < 64 ? g * G : 0),
+                            [&](int32_t col, float prod) {
+                                const int s = hash_find<H>(T, col, err);
+                                if (s >= 0) atomicAdd(&V[s], prod);
+                            });
     }
     __syncthreads();
     for (int q = cs + lane; q < ce; q += G) {
@@ -120,6 +132,7 @@ __global__ __launch_bounds__(R *G) void k_num_dense(const int32_t *__restrict__ 
                                                     NumArgs a)
 {
     __shared__ float accs[R * W];
+    __shared__ KMeta meta[G < 64 ? R * G : 1];
     const int g = threadIdx.x / G, lane = threadIdx.x % G;
     const int64_t ridx = (int64_t)blockIdx.x * R + g;
     const bool valid = ridx < nrows;
@@ -130,13 +143,14 @@ __global__ __launch_bounds__(R *G) void k_num_dense(const int32_t *__restrict__ 
     const int32_t wlo = valid ? a.lo[row] : 0;
     if (valid) {
         int *err = a.err;
-        walk_row_ordered<G>(a, a.arp[row], a.arp[row + 1], [&](int32_t col, float prod) {
-            const int32_t c = col - wlo;
-            if ((uint32_t)c < (uint32_t)W)
-                atomicAdd(&S[c], prod);
-            else
-                atomicOr(err, kErrWindow);
-        });
+        walk_row_ordered<G>(a, a.arp[row], a.arp[row + 1], meta + (G < 64 ? g * G : 0),
+                            [&](int32_t col, float prod) {
+                                const int32_t c = col - wlo;
+                                if ((uint32_t)c < (uint32_t)W)
+                                    atomicAdd(&S[c], prod);
+                                else
+                                    atomicOr(err, kErrWindow);
+                            });
     }
     __syncthreads();
     const int cs = valid ? a.crp[row] : 0, ce = valid ? a.crp[row + 1] : 0;

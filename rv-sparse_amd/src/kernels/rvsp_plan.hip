@@ -84,6 +84,17 @@ void dfree(void *p, hipStream_t st)
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// Rank-based reuse passes are on unless RVSP_REUSE_RANKS=0 (read once).
+bool reuse_ranks_off()
+{
+    static int off = -1;
+    if (off < 0) {
+        const char *e = getenv("RVSP_REUSE_RANKS");
+        off = e && e[0] == '0' ? 1 : 0;
+    }
+    return off == 1;
+}
+
 // Big-row numeric window shape: kBigShapes[0] unless RVSP_BIG_WINDOW names
 // another launchable one as "<outputs>x<chunks>" (read once).
 int big_window_shape()
@@ -169,13 +180,36 @@ struct rvsp_hip_plan {
     int32_t h_off[3][64];
     bool analysed = false;
 
+    // reused structure: per-product output ranks of the LDS-path rows
+    void *pmblock = nullptr;  // per-row product offsets (int64), scan scratch
+    uint16_t *pmap = nullptr;
+    int64_t *prow = nullptr;
+    int64_t pm_products = 0;
+    bool pm_ready = false, pm_skip = false;
+
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     const void *gkey[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
+static void drop_graph(rvsp_hip_plan *p)
+{
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->gexec = nullptr;
+    p->graph = nullptr;
+    for (auto &k : p->gkey) k = nullptr;
+}
+
 static void plan_release(rvsp_hip_plan *p)
 {
+    dfree(p->pmblock, p->stream);
+    dfree(p->pmap, p->stream);
+    p->pmblock = nullptr;
+    p->pmap = nullptr;
+    p->prow = nullptr;
+    p->pm_products = 0;
+    p->pm_ready = p->pm_skip = false;
     dfree(p->rowblock, p->stream);
     dfree(p->bigblock, p->stream);
     dfree(p->cursors, p->stream);
@@ -485,15 +519,71 @@ static NumArgs num_args(rvsp_hip_plan *p, const float *aval, const float *bval, 
     return NumArgs{p->arp, p->aci, aval, p->brp, p->bci, bval, p->crp, cci, cval, p->lo, p->err};
 }
 
+static void queue_numeric(rvsp_hip_plan *p, const float *aval, const float *bval,
+                          const int32_t *cci, float *cval)
+{
+    const NumArgs a = num_args(p, aval, bval, cci, cval);
+    if (p->pm_ready)
+        launch_numeric_ranked(p->stream, a, p->nperm, p->num_count, p->num_off, p->big, p->nseg,
+                              p->prow, p->pmap);
+    else
+        launch_numeric(p->stream, a, p->nperm, p->num_count, p->num_off, p->big, p->nseg);
+}
+
 rvsp_status_t rvsp_hip_plan_numeric(rvsp_hip_plan_t *p, const float *aval, const float *bval,
                                     const int32_t *cci, float *cval)
 {
     if (!p || !aval || !bval || !cci || !cval) return RVSP_ERROR_NULL_POINTER;
     if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
     PhaseScope timer(p->stream, PH_NUMERIC);
-    launch_numeric(p->stream, num_args(p, aval, bval, cci, cval), p->nperm, p->num_count,
-                   p->num_off, p->big, p->nseg);
+    queue_numeric(p, aval, bval, cci, cval);
     HIP_TRY(hipGetLastError());
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_plan_prepare_reuse(rvsp_hip_plan_t *p, const int32_t *cci)
+{
+    if (!p || !cci) return RVSP_ERROR_NULL_POINTER;
+    if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
+    if (p->pm_ready || p->pm_skip || reuse_ranks_off()) return RVSP_SUCCESS;
+    hipStream_t st = p->stream;
+    const size_t M = (size_t)p->m;
+    const int ntiles = pm_scan_tiles(p->m);
+    size_t off = 0;
+    const size_t o_prow = off; off = align256(off + M * 8);
+    const size_t o_cnt = off; off = align256(off + M * 8);
+    const size_t o_tsum = off; off = align256(off + (size_t)ntiles * 8);
+    const size_t o_tot = off; off = align256(off + 8);
+    rvsp_status_t s = dalloc(&p->pmblock, off, st);
+    if (s != RVSP_SUCCESS) {
+        p->pm_skip = true;  // the plain kernels stay in use
+        return RVSP_SUCCESS;
+    }
+    char *b = (char *)p->pmblock;
+    p->prow = (int64_t *)(b + o_prow);
+    int64_t *cnt = (int64_t *)(b + o_cnt), *tsum = (int64_t *)(b + o_tsum);
+    int64_t *tot = (int64_t *)(b + o_tot);
+    launch_pm_build(st, p->m, p->nbin, p->ops, p->arp, p->aci, p->brp, p->bci, p->crp, cci, cnt,
+                    tsum, tot, p->prow, nullptr, p->err);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(&p->pm_products, tot, 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // 2 bytes per product; keep to a quarter of the free device memory
+    size_t free_b = 0, total_b = 0;
+    const size_t need = (size_t)p->pm_products * 2;
+    if (p->pm_products <= 0 || hipMemGetInfo(&free_b, &total_b) != hipSuccess || need > free_b / 4 ||
+        dalloc((void **)&p->pmap, need, st) != RVSP_SUCCESS) {
+        dfree(p->pmblock, st);
+        p->pmblock = nullptr;
+        p->prow = nullptr;
+        p->pm_skip = true;
+        return RVSP_SUCCESS;
+    }
+    launch_pm_build(st, p->m, p->nbin, p->ops, p->arp, p->aci, p->brp, p->bci, p->crp, cci, cnt,
+                    tsum, tot, p->prow, p->pmap, p->err);
+    HIP_TRY(hipGetLastError());
+    p->pm_ready = true;
+    drop_graph(p);  // a captured pass would still use the plain kernels
     return RVSP_SUCCESS;
 }
 
@@ -511,8 +601,7 @@ rvsp_status_t rvsp_hip_plan_numeric_graph(rvsp_hip_plan_t *p, const float *aval,
     }
     if (!p->gexec) {
         HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-        launch_numeric(p->stream, num_args(p, aval, bval, cci, cval), p->nperm, p->num_count,
-                       p->num_off, p->big, p->nseg);
+        queue_numeric(p, aval, bval, cci, cval);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(p->stream, &g);
         if (e != hipSuccess) return map_err(e);
@@ -581,11 +670,7 @@ rvsp_status_t rvsp_hip_plan_set_num_bins(rvsp_hip_plan_t *p, const uint8_t *num_
     launch_bin_scatter(st, p->m, p->nbin, p->binoff, p->binoff + 64, p->nperm, NB_COUNT);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
-    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
-    if (p->graph) (void)hipGraphDestroy(p->graph);
-    p->gexec = nullptr;
-    p->graph = nullptr;
-    for (auto &k : p->gkey) k = nullptr;
+    drop_graph(p);
     return RVSP_SUCCESS;
 }
 
