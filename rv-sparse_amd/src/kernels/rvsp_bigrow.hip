@@ -321,36 +321,40 @@ __device__ __forceinline__ int big_batch(float aj, int32_t col, float bv, int32_
 
 // Adds the products of one A entry's piece in [c0, c1) that start at `cj`
 // (the first 64 entries already loaded in col/val); returns the new
-// cursor. All adds of a piece have the same k and distinct columns, so
+// cursor and the column found there. All adds of a piece have the same k and distinct columns, so
 // their order does not matter: a piece longer than 64 entries has its next
 // kPieceBatches batches loaded at once before their adds.
 constexpr int kPieceBatches = 4;
 
-__device__ __forceinline__ int big_piece(int cj, int ej, float aj, int32_t col, float bv,
-                                         int32_t c0, int32_t c1, int outs, const uint32_t *W,
-                                         const uint16_t *P, float *acc, const NumArgs &a)
+__device__ __forceinline__ int2 big_piece(int cj, int ej, float aj, int32_t col, float bv,
+                                          int32_t c0, int32_t c1, int outs, const uint32_t *W,
+                                          const uint16_t *P, float *acc, const NumArgs &a)
 {
     const int lane = threadIdx.x & 63;
     int nin = big_batch(aj, col, bv, c0, c1, outs, W, P, acc, a);
     cj += nin;
+    int32_t last = col;  // the batch whose first out-of-window entry ends the piece
     while (nin == 64) {
         int32_t cb[kPieceBatches];
         float vb[kPieceBatches];
 #pragma unroll
         for (int b = 0; b < kPieceBatches; ++b) {
             const int bp = cj + b * 64 + lane;
-            cb[b] = bp < ej ? a.bci[bp] : INT32_MAX;
-            vb[b] = bp < ej ? a.bval[bp] : 0.0f;
+            const int2 cv = bp < ej ? a.bcv[bp] : make_int2(INT32_MAX, 0);
+            cb[b] = cv.x;
+            vb[b] = __int_as_float(cv.y);
         }
 #pragma unroll
         for (int b = 0; b < kPieceBatches; ++b) {
             if (nin == 64) {  // uniform: the previous batch lay wholly in the window
                 nin = big_batch(aj, cb[b], vb[b], c0, c1, outs, W, P, acc, a);
                 cj += nin;
+                last = cb[b];
             }
         }
     }
-    return cj;
+    // new cursor, and the column there (INT32_MAX past the end of B_k)
+    return make_int2(cj, lane_bcast(last, nin));
 }
 
 // Up to four live A entries of a window, taken from a lane mask in lane
@@ -361,7 +365,7 @@ struct PieceGroup {
     float av[4], bv[4];
     int32_t col[4];
 
-    __device__ __forceinline__ void take(uint64_t &mask, int2 ce, float a_val, const NumArgs &a)
+    __device__ __forceinline__ void take(uint64_t &mask, int4 ce, float a_val, const NumArgs &a)
     {
         const int lane = threadIdx.x & 63;
         n = 0;
@@ -381,8 +385,9 @@ struct PieceGroup {
             av[u] = lane_bcast(a_val, j[u]);
             const int bp = c[u] + lane;
             const bool ok = u < n && bp < e[u];
-            col[u] = ok ? a.bci[bp] : INT32_MAX;
-            bv[u] = ok ? a.bval[bp] : 0.0f;
+            const int2 cv = ok ? a.bcv[bp] : make_int2(INT32_MAX, 0);
+            col[u] = cv.x;
+            bv[u] = __int_as_float(cv.y);
         }
     }
 };
@@ -402,7 +407,7 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
     uint16_t *P = P_s[wv];
     const int4 sg = g.segs[it];
     const int row = sg.x, w0 = sg.y, w1 = sg.z;
-    int2 *cur_a = g.cursors + g.seg_cur[it];
+    int4 *cur_a = g.cursors + g.seg_cur[it];
     const int as = a.arp[row], ae = a.arp[row + 1];
     const int32_t rlo = g.lo[row], rhi = g.hi[row];
     const int nchunks = (int)(((int64_t)rhi - rlo + kChunkCols) / kChunkCols);
@@ -412,8 +417,8 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
     const int rowbase = a.crp[row];
     const int nz = a.crp[row + 1] - rowbase;
 
-    // per-entry state at the segment's first column: (cursor, end of B_k);
-    // four binary searches per lane run interleaved
+    // per-entry state at the segment's first column: (cursor, end of B_k,
+    // column at the cursor); four binary searches per lane run interleaved
     const int32_t cstart = rlo + g.win_chunk0[c32 + w0] * kChunkCols;
     for (int base = as; base < ae; base += 4 * 64) {
         int lo4[4], hi4[4], e4[4];
@@ -446,7 +451,8 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int ap = base + u * 64 + lane;
-            if (ap < ae) cur_a[ap - as] = make_int2(lo4[u], e4[u]);
+            const int32_t nc = ap < ae && lo4[u] < e4[u] ? a.bci[lo4[u]] : INT32_MAX;
+            if (ap < ae) cur_a[ap - as] = make_int4(lo4[u], e4[u], nc, 0);
         }
     }
 
@@ -489,15 +495,15 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
 
         for (int base = as; base < ae; base += 64) {
             const int ap = base + lane;
-            int2 ce = make_int2(0, 0);
+            int4 ce = make_int4(0, 0, INT32_MAX, 0);
             float av = 0.0f;
-            bool live = false;
             if (ap < ae) {
                 ce = cur_a[ap - as];
                 av = a.aval[ap];
-                live = ce.x < ce.y && a.bci[ce.x] < c1;
             }
-            int cur = ce.x;
+            // the cursor carries the column it points at: no load to test it
+            const bool live = ce.x < ce.y && ce.z < c1;
+            int cur = ce.x, nxt = ce.z;
             uint64_t mask = __ballot(live);
             while (mask) {
                 // up to four live entries: issue all their loads first
@@ -506,14 +512,17 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     if (u < g0.n) {
-                        const int nc = big_piece(g0.c[u], g0.e[u], g0.av[u], g0.col[u], g0.bv[u],
-                                                 c0, c1, outs, W, P, acc, a);
+                        const int2 nc = big_piece(g0.c[u], g0.e[u], g0.av[u], g0.col[u],
+                                                  g0.bv[u], c0, c1, outs, W, P, acc, a);
                         compiler_fence();
-                        if (lane == g0.j[u]) cur = nc;
+                        if (lane == g0.j[u]) {
+                            cur = nc.x;
+                            nxt = nc.y;
+                        }
                     }
                 }
             }
-            if (ap < ae) cur_a[ap - as].x = cur;
+            if (live) cur_a[ap - as] = make_int4(cur, ce.y, nxt, 0);
         }
         wave_sync();
         for (int i = lane; i < outs; i += 64) a.cval[rowbase + o0 + i] = acc[i];

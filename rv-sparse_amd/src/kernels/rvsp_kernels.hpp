@@ -15,7 +15,7 @@ struct AnalysisCounters {
     unsigned long long products;
     unsigned long long bits_cap;   // sum over big rows of bitmap words
     unsigned long long bits_fill;  // running allocation of bitmap words
-    unsigned long long cur_total;  // cursor ints over all big-row segments
+    unsigned long long cur_total;  // cursors over all big-row segments
     int nseg;                      // big-row segments
     int moved;                     // learned sizing: rows in a smaller table
     long long nnz_total;           // scan total
@@ -36,7 +36,8 @@ struct BigArgs {
     int32_t *big_nwin;    // per row: number of windows
     int4 *segs;           // (row, first window, end window, 0)
     int64_t *seg_cur;     // per segment: offset of its cursor block
-    int2 *cursors;        // per (segment, A entry): (next B position, end of B row)
+    int4 *cursors;        // per (segment, A entry): (next B position, end of B row,
+                          // column at that position or INT32_MAX, 0)
     int pass_words;       // symbolic bitmap words per pass (LDS), multiple of kChunkWords
     int win_outs;         // numeric window: at most this many outputs ...
     int win_chunks;       // ... and this many chunks (one of the launchable shapes)
@@ -61,6 +62,7 @@ struct NumArgs {
     const float *aval;
     const int32_t *brp, *bci;
     const float *bval;
+    const int2 *bcv;      // B as (column, value bits) pairs: one 8-byte load per entry
     const int32_t *crp, *cci;
     float *cval;
     const int32_t *lo;
@@ -89,6 +91,9 @@ void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_
                      AnalysisCounters *ctr);
 void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
                     const int32_t *off, const BigArgs &big, int nsegs);
+// Writes B's (column, value) pairs for a numeric pass: bcv[i] = (bci[i], bval[i]).
+void launch_interleave(hipStream_t st, int32_t nnz, const int32_t *bci, const float *bval,
+                       int2 *bcv);
 // One hash or dense numeric bin over the given rows (false: not such a bin).
 bool launch_num_bin(hipStream_t st, const NumArgs &a, int bin, const int32_t *rows, int nrows);
 void launch_sym_big(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g);

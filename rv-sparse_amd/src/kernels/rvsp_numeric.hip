@@ -28,7 +28,8 @@ namespace rvsp {
 // calls add(col, fl(a*b)) for every product. Loads for four k-steps are
 // issued before any of their adds, so up to eight loads per lane are in
 // flight; the adds of each k-step are then issued in k order, separated by
-// compiler fences (B is read-only, so hoisting its loads is safe). The
+// compiler fences (B is read-only, so hoisting its loads is safe); B is read
+// as (column, value) pairs, one 8-byte load per entry. The
 // entry of a k-step reaches the group's lanes through a scalar register
 // (G = 64: one row per wave) or through the group's LDS table M of G
 // entries (one 16-byte load per k-step instead of lane permutes).
@@ -71,15 +72,18 @@ __device__ __forceinline__ void walk_row_ordered(const NumArgs &a, int as, int a
                 }
                 const int bp = su[u] + lane;
                 const bool ok = j0 + u < nch && bp < eu[u];
-                cu[u] = ok ? a.bci[bp] : -1;
-                vu[u] = ok ? a.bval[bp] : 0.0f;
+                const int2 cv = ok ? a.bcv[bp] : make_int2(-1, 0);
+                cu[u] = cv.x;
+                vu[u] = __int_as_float(cv.y);
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 if (j0 + u < nch) {  // uniform within the group
                     if (cu[u] >= 0) add(cu[u], __fmul_rn(au[u], vu[u]));
-                    for (int bp = su[u] + G + lane; bp < eu[u]; bp += G)
-                        add(a.bci[bp], __fmul_rn(au[u], a.bval[bp]));
+                    for (int bp = su[u] + G + lane; bp < eu[u]; bp += G) {
+                        const int2 cv = a.bcv[bp];
+                        add(cv.x, __fmul_rn(au[u], __int_as_float(cv.y)));
+                    }
                     compiler_fence();
                 }
             }
@@ -237,6 +241,20 @@ bool launch_num_bin(hipStream_t st, const NumArgs &a, int bin, const int32_t *ro
     default:
         return false;
     }
+}
+
+__global__ __launch_bounds__(256) void k_interleave(int32_t nnz, const int32_t *__restrict__ bci,
+                                                    const float *__restrict__ bval,
+                                                    int2 *__restrict__ bcv)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < nnz) bcv[i] = make_int2(bci[i], __float_as_int(bval[i]));
+}
+
+void launch_interleave(hipStream_t st, int32_t nnz, const int32_t *bci, const float *bval,
+                       int2 *bcv)
+{
+    if (nnz > 0) k_interleave<<<(int)(((int64_t)nnz + 255) / 256), 256, 0, st>>>(nnz, bci, bval, bcv);
 }
 
 void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,

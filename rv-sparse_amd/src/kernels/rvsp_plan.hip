@@ -166,7 +166,7 @@ struct rvsp_hip_plan {
     int *err = nullptr;
 
     void *bigblock = nullptr;  // big-row bitmaps, chunk counts, windows, segments
-    int2 *cursors = nullptr;
+    int4 *cursors = nullptr;
     BigArgs big{};
     int nseg = 0;
 
@@ -179,6 +179,8 @@ struct rvsp_hip_plan {
     // each stays untouched until the stream has passed its copy
     int32_t h_off[3][64];
     bool analysed = false;
+
+    int2 *bcv = nullptr;  // B's (column, value) pairs, rewritten by every numeric pass
 
     // reused structure: per-product output ranks of the LDS-path rows
     void *pmblock = nullptr;  // per-row product offsets (int64), scan scratch
@@ -203,6 +205,8 @@ static void drop_graph(rvsp_hip_plan *p)
 
 static void plan_release(rvsp_hip_plan *p)
 {
+    dfree(p->bcv, p->stream);
+    p->bcv = nullptr;
     dfree(p->pmblock, p->stream);
     dfree(p->pmap, p->stream);
     p->pmblock = nullptr;
@@ -470,7 +474,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->nnzc = p->hctr.nnz_total;
     p->nseg = p->hctr.nseg;
     if (p->nseg > 0) {
-        s = dalloc((void **)&p->cursors, (size_t)p->hctr.cur_total * 8, st);
+        s = dalloc((void **)&p->cursors, (size_t)p->hctr.cur_total * sizeof(int4), st);
         if (s != RVSP_SUCCESS) {
             plan_release(p);
             return s;
@@ -498,6 +502,11 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     if (c_nnz) *c_nnz = p->nnzc;
     if (products) *products = p->products;
     if (p->nnzc > INT32_MAX) return RVSP_ERROR_ALLOCATION_FAILED;
+    s = dalloc((void **)&p->bcv, (size_t)nnz_b * sizeof(int2), st);
+    if (s != RVSP_SUCCESS) {
+        plan_release(p);
+        return s;
+    }
     p->analysed = true;
     return RVSP_SUCCESS;
 }
@@ -516,12 +525,17 @@ rvsp_status_t rvsp_hip_plan_fill(rvsp_hip_plan_t *p, int32_t *cci)
 static NumArgs num_args(rvsp_hip_plan *p, const float *aval, const float *bval, const int32_t *cci,
                         float *cval)
 {
-    return NumArgs{p->arp, p->aci, aval, p->brp, p->bci, bval, p->crp, cci, cval, p->lo, p->err};
+    return NumArgs{p->arp, p->aci, aval,    p->brp, p->bci, bval,
+                   p->bcv, p->crp, cci,     cval,   p->lo,  p->err};
 }
 
 static void queue_numeric(rvsp_hip_plan *p, const float *aval, const float *bval,
                           const int32_t *cci, float *cval)
 {
+    // kernels that gather B's columns read the (column, value) pairs; the
+    // rank kernels read only values, so a pass with ranks and no big rows
+    // skips the pairs
+    if (!p->pm_ready || p->nseg > 0) launch_interleave(p->stream, p->nnz_b, p->bci, bval, p->bcv);
     const NumArgs a = num_args(p, aval, bval, cci, cval);
     if (p->pm_ready)
         launch_numeric_ranked(p->stream, a, p->nperm, p->num_count, p->num_off, p->big, p->nseg,
@@ -710,6 +724,7 @@ rvsp_status_t rvsp_hip_plan_time_bin(rvsp_hip_plan_t *p, int32_t bin, const int3
         return RVSP_ERROR_UNSUPPORTED_BACKEND;
     }
     rvsp_status_t s = RVSP_SUCCESS;
+    launch_interleave(st, p->nnz_b, p->bci, bval, p->bcv);
     (void)launch_num_bin(st, a, bin, rows, nrows);  // warm-up
     (void)hipEventRecord(e0, st);
     for (int r = 0; r < reps; ++r) (void)launch_num_bin(st, a, bin, rows, nrows);
