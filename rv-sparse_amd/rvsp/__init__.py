@@ -454,7 +454,7 @@ def spgemm_batched(dAs, dBs, algo=ALGO_DEFAULT, fetch=True, with_nnz=False):
 # ---------------------------------------------------------------------------
 SELECT_HEURISTIC, SELECT_LEARNED = 0, 1
 SELECTOR_SIZER, SELECTOR_CHOOSER = 0, 1
-NB_HASH0, NB_DENSE0, NB_WIN, NB_COUNT = 1, 29, 53, 54
+NB_HASH0, NB_DENSE0, NB_RANK0, NB_WIN, NB_COUNT = 1, 29, 53, 61, 62
 SB_EMPTY, SB_H64, SB_BMS, SB_BIG = 0, 1, 8, 9
 
 

@@ -29,7 +29,7 @@ import time
 
 import numpy as np
 
-from . import (ALGO_HIP, DeviceBuffer, DeviceCsr, NB_DENSE0, NB_HASH0, NB_WIN, Plan,
+from . import (ALGO_HIP, DeviceBuffer, DeviceCsr, NB_DENSE0, NB_HASH0, NB_RANK0, NB_WIN, Plan,
                SELECT_HEURISTIC, SELECT_LEARNED, SELECTOR_CHOOSER, SELECTOR_SIZER, Csr,
                gen_banded, gen_random_rows, gen_rmat, gen_stencil27, gen_zipf_rows,
                get_timing, reset_timing, selector_clear, selector_param_count,
@@ -38,6 +38,8 @@ from . import (ALGO_HIP, DeviceBuffer, DeviceCsr, NB_DENSE0, NB_HASH0, NB_WIN, P
 NF_NO = {SELECTOR_SIZER: (4, 1), SELECTOR_CHOOSER: (5, 8)}
 HASH_MAX_NNZ = 2730
 DENSE_MAX_COLS = 16384
+RANK_MAX_COLS = 16384
+RANK_MAX_NNZ = 4096
 
 
 def training_products():
@@ -104,7 +106,8 @@ def _next_pow2(v):
 
 def chooser_bins(nz, span):
     """(m, 8) numeric bin of each chooser output per row, -1 if infeasible;
-    the host image of chooser_bin in rvsp_mlp.hpp."""
+    the host image of chooser_bin in rvsp_mlp.hpp (outputs 0-3 hash table,
+    4-7 window: a rank window where it fits with 32/64 lanes, else dense)."""
     nz = np.asarray(nz, np.int64)
     span = np.asarray(span, np.int64)
     H = np.maximum(64, _next_pow2((3 * nz + 1) // 2))
@@ -121,7 +124,11 @@ def chooser_bins(nz, span):
         else:
             gmin = np.where(W >= 16384, 64, np.where(W >= 8192, 32, np.where(W >= 4096, 16, 8)))
             ok = (span <= DENSE_MAX_COLS) & (G >= gmin)
-            out[:, o] = np.where(ok, NB_DENSE0 + 4 * (lw - 9) + (o & 3), -1)
+            dense = np.where(ok, NB_DENSE0 + 4 * (lw - 9) + (o & 3), -1)
+            C = np.maximum(512, _next_pow2(nz))
+            lc = np.log2(C).astype(np.int64)
+            rank_ok = (span <= RANK_MAX_COLS) & (nz <= RANK_MAX_NNZ) & (G >= 32)
+            out[:, o] = np.where(rank_ok, NB_RANK0 + 2 * (lc - 9) + (1 if G == 64 else 0), dense)
     return out
 
 

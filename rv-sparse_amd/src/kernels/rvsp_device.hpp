@@ -30,12 +30,16 @@ enum SymBin : int {
 // Numeric bins: 0 empty; NB_HASH0 + 4*(log2(H)-6) + (log2(G)-3) for LDS
 // hash tables of H slots worked by G lanes; NB_DENSE0 + 4*(log2(W)-9) +
 // (log2(G)-3) for dense windows of W columns over the row's product span;
-// NB_WIN for big rows (rvsp_bigrow.hip).
+// NB_RANK0 + 2*(log2(CAP)-9) + (G == 64) for rank windows (a bitmap of C_i's
+// columns over a span of at most kRankMaxCols, its popcount prefix, and an
+// accumulator of CAP >= nnz(C_i) output positions; G = 32 or 64); NB_WIN
+// for big rows (rvsp_bigrow.hip).
 constexpr int NB_EMPTY = 0;
 constexpr int NB_HASH0 = 1;
 constexpr int NB_DENSE0 = NB_HASH0 + 7 * 4;  // 29
-constexpr int NB_WIN = NB_DENSE0 + 6 * 4;    // 53
-constexpr int NB_COUNT = NB_WIN + 1;         // 54
+constexpr int NB_RANK0 = NB_DENSE0 + 6 * 4;  // 53
+constexpr int NB_WIN = NB_RANK0 + 4 * 2;     // 61
+constexpr int NB_COUNT = NB_WIN + 1;         // 62
 // bin scatter keeps per-bin offsets and cursors in 64-entry regions
 static_assert(NB_COUNT <= 64 && SB_COUNT <= 64, "bin tables hold 64 bins");
 
@@ -46,6 +50,9 @@ constexpr int kChunkWords = kChunkCols / 32;
 constexpr int kSegProducts = 16384;       // products per big-row segment (one wave)
 constexpr int kDenseMaxCols = 16384;      // widest dense window (64 KB acc)
 constexpr int kHashMaxNnz = 2730;         // numeric hash rows: H = 4096 at load <= 2/3
+constexpr int kRankMaxCols = 16384;       // rank windows: span (512 bitmap words)
+constexpr int kRankWords = kRankMaxCols / 32;
+constexpr int kRankMaxNnz = 4096;         // rank windows: outputs
 constexpr int kSymHashMaxU = 2048;        // symbolic hash rows: min(ops,span) <= this
 
 __device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }

@@ -67,7 +67,9 @@ __device__ __forceinline__ void row_features(int64_t ops, int alen, int64_t span
 }
 
 // Numeric bin of chooser output o for a row, or -1 when not feasible
-// (only the instantiated (H, G) and (W, G) pairs are).
+// (only the instantiated (H, G), (W, G) and (CAP, G) triples are). Outputs
+// 0-3: hash table; 4-7: window (rank window when the row fits one and G is
+// 32 or 64, else dense window).
 __device__ __forceinline__ int chooser_bin(int o, int nz, int64_t span)
 {
     const int lg = 3 + (o & 3);
@@ -76,6 +78,11 @@ __device__ __forceinline__ int chooser_bin(int o, int nz, int64_t span)
         const int H = max(64, next_pow2((3 * nz + 1) / 2));
         if ((1 << lg) < (H >= 4096 ? 32 : H >= 2048 ? 16 : 8)) return -1;
         return NB_HASH0 + 4 * (ilog2(H) - 6) + (lg - 3);
+    }
+    // window kind: a rank window where it fits (32 or 64 lanes), else dense
+    if (span <= kRankMaxCols && nz <= kRankMaxNnz && lg >= 5) {
+        const int C = max(512, next_pow2(nz));
+        return NB_RANK0 + 2 * (ilog2(C) - 9) + (lg == 6 ? 1 : 0);
     }
     if (span > kDenseMaxCols) return -1;
     const int W = max(512, next_pow2((int)span));

@@ -164,6 +164,93 @@ __global__ __launch_bounds__(R *G) void k_num_dense(const int32_t *__restrict__ 
     }
 }
 
+// Rank window: G lanes per row (32 or 64), R rows per block. The row's
+// output columns (C_i's structure) are set in an LDS bitmap over its product
+// span, each word gets the popcount of the words before it, and a product
+// at column c adds into acc[rank(c)] (two LDS reads and a popcount). The
+// accumulator holds CAP >= nnz(C_i) output positions, so it drains straight
+// into C_i's values. Same ordered walk as the hash kernel.
+template <int G, int CAP, int R>
+__global__ __launch_bounds__(R *G) void k_num_rankw(const int32_t *__restrict__ rows, int32_t nrows,
+                                                    NumArgs a)
+{
+    constexpr int WPL = kRankWords / G;  // bitmap words per lane in the prefix
+    __shared__ uint32_t Wd[R * kRankWords];
+    __shared__ uint16_t Pd[R * kRankWords];
+    __shared__ float accs[R * CAP];
+    __shared__ KMeta meta[G < 64 ? R * G : 1];
+    const int g = threadIdx.x / G, lane = threadIdx.x % G;
+    const int64_t ridx = (int64_t)blockIdx.x * R + g;
+    const bool valid = ridx < nrows;
+    const int row = valid ? rows[ridx] : 0;
+    uint32_t *Wr = Wd + g * kRankWords;
+    uint16_t *Pr = Pd + g * kRankWords;
+    float *S = accs + g * CAP;
+    const int cs = valid ? a.crp[row] : 0;
+    const int nz = valid ? min(a.crp[row + 1] - cs, CAP) : 0;
+    const int32_t wlo = valid ? a.lo[row] : 0;
+    for (int i = lane; i < kRankWords; i += G) Wr[i] = 0u;
+    for (int i = lane; i < nz; i += G) S[i] = 0.0f;
+    __syncthreads();
+    for (int q = lane; q < nz; q += G) {
+        const int32_t c = a.cci[cs + q] - wlo;
+        if ((uint32_t)c < (uint32_t)kRankMaxCols)
+            atomicOr(&Wr[c >> 5], 1u << (c & 31));
+        else
+            atomicOr(a.err, kErrWindow);
+    }
+    __syncthreads();
+    {
+        int local = 0;
+#pragma unroll
+        for (int j = 0; j < WPL; ++j) local += __popc(Wr[lane * WPL + j]);
+        int incl = local;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+            const int y = __shfl_up(incl, o, G);
+            if (lane >= o) incl += y;
+        }
+        int run = incl - local;
+#pragma unroll
+        for (int j = 0; j < WPL; ++j) {
+            Pr[lane * WPL + j] = (uint16_t)run;
+            run += __popc(Wr[lane * WPL + j]);
+        }
+    }
+    __syncthreads();
+    if (valid) {
+        int *err = a.err;
+        walk_row_ordered<G>(a, a.arp[row], a.arp[row + 1], meta + (G < 64 ? g * G : 0),
+                            [&](int32_t col, float prod) {
+                                const int32_t c = col - wlo;
+                                if ((uint32_t)c < (uint32_t)kRankMaxCols) {
+                                    const int wi = c >> 5;
+                                    const uint32_t word = Wr[wi];
+                                    const int pos =
+                                        (int)Pr[wi] + __popc(word & ((1u << (c & 31)) - 1u));
+                                    if (((word >> (c & 31)) & 1u) && pos < nz)
+                                        atomicAdd(&S[pos], prod);
+                                    else
+                                        atomicOr(err, kErrProbeMiss);
+                                } else {
+                                    atomicOr(err, kErrWindow);
+                                }
+                            });
+    }
+    __syncthreads();
+    for (int q = lane; q < nz; q += G) a.cval[cs + q] = S[q];
+}
+
+template <int G, int CAP>
+static void launch_rankw(hipStream_t st, const NumArgs &a, const int32_t *rows, int nrows)
+{
+    constexpr int ROWB = kRankWords * 6 + CAP * 4 + (G < 64 ? G * 16 : 0);
+    constexpr int RMAX = (256 / G) < (49152 / ROWB) ? (256 / G) : (49152 / ROWB);
+    constexpr int R = (G == 32 && RMAX > 1) ? (RMAX & ~1) : (RMAX > 0 ? RMAX : 1);
+    static_assert((R * G) % 64 == 0, "block must be whole waves");
+    k_num_rankw<G, CAP, R><<<(nrows + R - 1) / R, R * G, 0, st>>>(rows, nrows, a);
+}
+
 template <int G, int H>
 static void launch_hash(hipStream_t st, const NumArgs &a, const int32_t *rows, int nrows)
 {
@@ -185,6 +272,10 @@ static void launch_dense(hipStream_t st, const NumArgs &a, const int32_t *rows, 
 #define RVSP_HASH_CASE(HI, GI, H, G)                                                            \
     case NB_HASH0 + 4 * (HI) + (GI):                                                            \
         launch_hash<G, H>(st, a, rows, nr);                                                     \
+        return true;
+#define RVSP_RANK_CASE(CI, CAP, G)                                                              \
+    case NB_RANK0 + 2 * (CI) + ((G) == 64 ? 1 : 0):                                             \
+        launch_rankw<G, CAP>(st, a, rows, nr);                                                  \
         return true;
 #define RVSP_DENSE_CASE(WI, GI, W, G)                                                           \
     case NB_DENSE0 + 4 * (WI) + (GI):                                                           \
@@ -238,6 +329,14 @@ bool launch_num_bin(hipStream_t st, const NumArgs &a, int bin, const int32_t *ro
         RVSP_DENSE_CASE(4, 2, 8192, 32)
         RVSP_DENSE_CASE(4, 3, 8192, 64)
         RVSP_DENSE_CASE(5, 3, 16384, 64)
+        RVSP_RANK_CASE(0, 512, 32)
+        RVSP_RANK_CASE(0, 512, 64)
+        RVSP_RANK_CASE(1, 1024, 32)
+        RVSP_RANK_CASE(1, 1024, 64)
+        RVSP_RANK_CASE(2, 2048, 32)
+        RVSP_RANK_CASE(2, 2048, 64)
+        RVSP_RANK_CASE(3, 4096, 32)
+        RVSP_RANK_CASE(3, 4096, 64)
     default:
         return false;
     }

@@ -248,11 +248,16 @@ static void launch_ranked(hipStream_t st, const NumArgs &a, const int32_t *rows,
     k_num_ranked<G, CAP, R><<<(nrows + R - 1) / R, R * G, 0, st>>>(rows, nrows, a, prow, pmap);
 }
 
-// The hash bin (H, G) and the dense bin (W, G) hold rows of at most H and W
-// outputs: the rank accumulator takes that many floats.
+// The hash bin (H, G), the dense bin (W, G) and the rank-window bin (CAP, G)
+// hold rows of at most H, W and CAP outputs: the rank accumulator takes that
+// many floats.
 #define RVSP_RANK_HASH(HI, GI, H, G)                                                            \
     case NB_HASH0 + 4 * (HI) + (GI):                                                            \
         launch_ranked<G, H>(st, a, rows, nr, prow, pmap);                                       \
+        return true;
+#define RVSP_RANK_WIN(CI, CAP, G)                                                               \
+    case NB_RANK0 + 2 * (CI) + ((G) == 64 ? 1 : 0):                                             \
+        launch_ranked<G, CAP>(st, a, rows, nr, prow, pmap);                                     \
         return true;
 #define RVSP_RANK_DENSE(WI, GI, W, G)                                                           \
     case NB_DENSE0 + 4 * (WI) + (GI):                                                           \
@@ -306,6 +311,14 @@ static bool launch_ranked_bin(hipStream_t st, const NumArgs &a, int bin, const i
         RVSP_RANK_DENSE(4, 2, 8192, 32)
         RVSP_RANK_DENSE(4, 3, 8192, 64)
         RVSP_RANK_DENSE(5, 3, 16384, 64)
+        RVSP_RANK_WIN(0, 512, 32)
+        RVSP_RANK_WIN(0, 512, 64)
+        RVSP_RANK_WIN(1, 1024, 32)
+        RVSP_RANK_WIN(1, 1024, 64)
+        RVSP_RANK_WIN(2, 2048, 32)
+        RVSP_RANK_WIN(2, 2048, 64)
+        RVSP_RANK_WIN(3, 4096, 32)
+        RVSP_RANK_WIN(3, 4096, 64)
     default:
         return false;
     }

@@ -423,14 +423,25 @@ __device__ __forceinline__ int lanes_for(int64_t ops, int alen, int rows_per_blo
     return max(G, gmin);
 }
 
-// Hash table of H >= 1.5 nnz slots (8 bytes each) or a dense window of
-// W >= span columns (4 bytes each): the dense window wins when it is at
-// most twice the table's bytes, since it needs no probing at all.
+// Candidate accumulators, by LDS bytes per row: a hash table of H >= 1.5 nnz
+// slots (8 bytes each; counted twice, for its probing), a dense window of
+// W >= span columns (4 bytes each), or a rank window (a 512-word bitmap over
+// a span of at most 16384 columns with its 16-bit popcount prefix, 3 KB,
+// plus 4 bytes per output position, CAP >= nnz). The cheapest feasible one
+// wins.
 __device__ __forceinline__ int num_bin_of(int nz, int64_t ops, int alen, int64_t span)
 {
     const int H = nz <= kHashMaxNnz ? max(64, next_pow2((3 * nz + 1) / 2)) : 0;
     const int Wd = span <= kDenseMaxCols ? max(512, next_pow2((int)span)) : 0;
-    if (Wd && (!H || 4 * Wd <= 2 * 8 * H)) {
+    const int C = span <= kRankMaxCols && nz <= kRankMaxNnz ? max(512, next_pow2(nz)) : 0;
+    const int64_t fh = H ? 16LL * H : INT64_MAX;
+    const int64_t fd = Wd ? 4LL * Wd : INT64_MAX;
+    const int64_t fr = C ? kRankWords * 6LL + 4LL * C : INT64_MAX;
+    if (fr <= fh && fr <= fd) {
+        const int G = max(32, lanes_for(ops, alen, 4));
+        return NB_RANK0 + 2 * (ilog2(C) - 9) + (G == 64 ? 1 : 0);
+    }
+    if (fd <= fh) {
         const int G = lanes_for(ops, alen, 16384 / Wd);
         return NB_DENSE0 + 4 * (ilog2(Wd) - 9) + (ilog2(G) - 3);
     }

@@ -107,6 +107,11 @@ def brute_chooser_bin(o, nz, span):
         if G < (32 if H >= 4096 else 16 if H >= 2048 else 8):
             return -1
         return 1 + 4 * (int(np.log2(H)) - 6) + (o & 3)
+    if span <= 16384 and nz <= 4096 and G >= 32:
+        C = 512
+        while C < nz:
+            C *= 2
+        return 53 + 2 * (int(np.log2(C)) - 9) + (1 if G == 64 else 0)
     if span > 16384:
         return -1
     W = 512
