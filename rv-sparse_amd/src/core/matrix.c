@@ -61,11 +61,13 @@ void rvsp_csr_destroy(rvsp_csr_matrix_t *A)
     }
     else if (A->owns_data == RVSP_OWNS_DEVICE)
     {
-        /* stream-ordered pool allocations (device one-shot) */
-        rvsp_hip_free_async(A->row_ptr, NULL);
-        rvsp_hip_free_async(A->col_idx, NULL);
-        rvsp_hip_free_async(A->values, NULL);
-        rvsp_hip_stream_sync(NULL);
+        /* stream-ordered pool allocations of the device one-shot, returned on
+           the stream they came from so the next call reuses them */
+        void *st = rvsp_hip_oneshot_stream();
+        rvsp_hip_free_async(A->row_ptr, st);
+        rvsp_hip_free_async(A->col_idx, st);
+        rvsp_hip_free_async(A->values, st);
+        rvsp_hip_stream_sync(st);
     }
 
     A->rows = 0;

@@ -46,6 +46,9 @@ static rvsp_status_t oneshot_host(const rvsp_csr_matrix_t *A, const rvsp_csr_mat
     s = rvsp_spgemm_set_algo(d, algo);
     size_t ws_bytes = 0;
     int32_t c_nnz = 0;
+    void *ost = rvsp_hip_oneshot_stream();
+    if (s == RVSP_SUCCESS && ost)
+        s = rvsp_spgemm_set_stream(d, ost);
     if (s == RVSP_SUCCESS)
         s = rvsp_spgemm_work_estimation(d, A, B, &ws_bytes, &c_nnz);
     if (s != RVSP_SUCCESS)
@@ -201,7 +204,9 @@ rvsp_status_t rvsp_spgemm_csr_device_algo(const rvsp_csr_matrix_t *A, const rvsp
     size_t ws_bytes = 0;
     int32_t c_nnz = 0;
     void *st = NULL;
+    void *ost = rvsp_hip_oneshot_stream();
     if ((s = rvsp_spgemm_set_algo(d, algo)) || (s = rvsp_spgemm_set_memory(d, RVSP_MEMORY_DEVICE)) ||
+        (ost && (s = rvsp_spgemm_set_stream(d, ost))) ||
         (s = rvsp_spgemm_work_estimation(d, A, B, &ws_bytes, &c_nnz)) ||
         (s = rvsp_spgemm_get_stream(d, &st)))
     {
@@ -229,10 +234,10 @@ rvsp_status_t rvsp_spgemm_csr_device_algo(const rvsp_csr_matrix_t *A, const rvsp
     rvsp_spgemm_descr_destroy(d);
     if (s != RVSP_SUCCESS)
     {
-        rvsp_hip_free_async(out.row_ptr, NULL);
-        rvsp_hip_free_async(out.col_idx, NULL);
-        rvsp_hip_free_async(out.values, NULL);
-        rvsp_hip_stream_sync(NULL);
+        rvsp_hip_free_async(out.row_ptr, st);
+        rvsp_hip_free_async(out.col_idx, st);
+        rvsp_hip_free_async(out.values, st);
+        rvsp_hip_stream_sync(st);
         return s;
     }
     *C = out;

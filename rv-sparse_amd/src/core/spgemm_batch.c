@@ -24,6 +24,10 @@ static double chunk_products(void)
     return v > 0.0 ? v : 1.2e9;
 }
 
+/* every allocation, copy and launch of a batch runs on the library's one
+   stream, so the pool reuses what the previous call freed */
+#define BST rvsp_hip_oneshot_stream()
+
 #define TRY(expr)                                                                                  \
     do                                                                                             \
     {                                                                                              \
@@ -61,9 +65,9 @@ static rvsp_status_t batch_push(struct rvsp_spgemm_batch *b, struct rvsp_batch_c
 
 static void free_chunk(struct rvsp_batch_chunk *c)
 {
-    rvsp_hip_free_async(c->rp, NULL);
-    rvsp_hip_free_async(c->ci, NULL);
-    rvsp_hip_free_async(c->va, NULL);
+    rvsp_hip_free_async(c->rp, BST);
+    rvsp_hip_free_async(c->ci, BST);
+    rvsp_hip_free_async(c->va, BST);
     memset(c, 0, sizeof(*c));
 }
 
@@ -73,7 +77,7 @@ void rvsp_spgemm_batch_destroy(rvsp_spgemm_batch_t b)
         return;
     for (int32_t i = 0; i < b->nchunks; i++)
         free_chunk(&b->chunks[i]);
-    rvsp_hip_stream_sync(NULL);
+    rvsp_hip_stream_sync(BST);
     free(b->chunks);
     free(b);
 }
@@ -115,12 +119,12 @@ static rvsp_status_t run_chunk(struct rvsp_spgemm_batch *batch, int32_t p0, int3
     if (!items || !row_off || !nrows || !bases)
         s = RVSP_ERROR_ALLOCATION_FAILED;
 
-    TRY(rvsp_hip_alloc_async(&arp, ((size_t)rows + 1) * 4, NULL));
-    TRY(rvsp_hip_alloc_async(&aci, (size_t)anz * 4, NULL));
-    TRY(rvsp_hip_alloc_async(&ava, (size_t)anz * 4, NULL));
-    TRY(rvsp_hip_alloc_async(&brp, ((size_t)brows + 1) * 4, NULL));
-    TRY(rvsp_hip_alloc_async(&bci, (size_t)bnz * 4, NULL));
-    TRY(rvsp_hip_alloc_async(&bva, (size_t)bnz * 4, NULL));
+    TRY(rvsp_hip_alloc_async(&arp, ((size_t)rows + 1) * 4, BST));
+    TRY(rvsp_hip_alloc_async(&aci, (size_t)anz * 4, BST));
+    TRY(rvsp_hip_alloc_async(&ava, (size_t)anz * 4, BST));
+    TRY(rvsp_hip_alloc_async(&brp, ((size_t)brows + 1) * 4, BST));
+    TRY(rvsp_hip_alloc_async(&bci, (size_t)bnz * 4, BST));
+    TRY(rvsp_hip_alloc_async(&bva, (size_t)bnz * 4, BST));
     if (s == RVSP_SUCCESS)
     {
         /* A_i columns shift by the B rows stacked before problem i */
@@ -136,7 +140,7 @@ static rvsp_status_t run_chunk(struct rvsp_spgemm_batch *batch, int32_t p0, int3
             no += M->nnz;
             ko += B[p0 + i].rows;
         }
-        s = rvsp_hip_pack(NULL, n, items, arp, aci, ava);
+        s = rvsp_hip_pack(BST, n, items, arp, aci, ava);
     }
     if (s == RVSP_SUCCESS)
     {
@@ -149,7 +153,7 @@ static rvsp_status_t run_chunk(struct rvsp_spgemm_batch *batch, int32_t p0, int3
             ro += M->rows;
             no += M->nnz;
         }
-        s = rvsp_hip_pack(NULL, n, items, brp, bci, bva);
+        s = rvsp_hip_pack(BST, n, items, brp, bci, bva);
     }
 
     rvsp_csr_matrix_t bigA = {(int32_t)rows, (int32_t)brows, (int32_t)anz, arp, aci, ava,
@@ -161,18 +165,20 @@ static rvsp_status_t run_chunk(struct rvsp_spgemm_batch *batch, int32_t p0, int3
     TRY(rvsp_spgemm_descr_create(&d));
     TRY(rvsp_spgemm_set_algo(d, algo));
     TRY(rvsp_spgemm_set_memory(d, RVSP_MEMORY_DEVICE));
+    if (rvsp_hip_oneshot_stream())
+        TRY(rvsp_spgemm_set_stream(d, rvsp_hip_oneshot_stream()));
     TRY(rvsp_spgemm_work_estimation(d, &bigA, &bigB, &ws, &cnnz));
     if (s == RVSP_ERROR_ALLOCATION_FAILED && n > 1)
     {
         /* C of this chunk overflows int32 (or memory ran out): halve it */
         rvsp_spgemm_descr_destroy(d);
         d = NULL;
-        rvsp_hip_free_async(arp, NULL);
-        rvsp_hip_free_async(aci, NULL);
-        rvsp_hip_free_async(ava, NULL);
-        rvsp_hip_free_async(brp, NULL);
-        rvsp_hip_free_async(bci, NULL);
-        rvsp_hip_free_async(bva, NULL);
+        rvsp_hip_free_async(arp, BST);
+        rvsp_hip_free_async(aci, BST);
+        rvsp_hip_free_async(ava, BST);
+        rvsp_hip_free_async(brp, BST);
+        rvsp_hip_free_async(bci, BST);
+        rvsp_hip_free_async(bva, BST);
         free(items);
         free(row_off);
         free(nrows);
@@ -184,22 +190,22 @@ static rvsp_status_t run_chunk(struct rvsp_spgemm_batch *batch, int32_t p0, int3
         return s;
     }
     const size_t cap = cnnz > 0 ? (size_t)cnnz : 1;
-    TRY(rvsp_hip_alloc_async(&crp, ((size_t)rows + 1) * 4, NULL));
-    TRY(rvsp_hip_alloc_async((void **)&ch.ci, cap * 4, NULL));
-    TRY(rvsp_hip_alloc_async((void **)&ch.va, cap * 4, NULL));
-    TRY(rvsp_hip_alloc_async((void **)&ch.rp, ((size_t)rows + n) * 4, NULL));
+    TRY(rvsp_hip_alloc_async(&crp, ((size_t)rows + 1) * 4, BST));
+    TRY(rvsp_hip_alloc_async((void **)&ch.ci, cap * 4, BST));
+    TRY(rvsp_hip_alloc_async((void **)&ch.va, cap * 4, BST));
+    TRY(rvsp_hip_alloc_async((void **)&ch.rp, ((size_t)rows + n) * 4, BST));
     if (s == RVSP_SUCCESS)
     {
         rvsp_csr_matrix_t bigC = {(int32_t)rows, bcols, cnnz, crp, ch.ci, ch.va,
                                   RVSP_DTYPE_FP32, RVSP_FORMAT_CSR, 0};
         char wsb[8];
-        rvsp_hip_stream_sync(NULL); /* allocations made on the null stream */
+        rvsp_hip_stream_sync(BST); /* allocations made on the library stream */
         s = rvsp_spgemm_compute(d, &bigA, &bigB, &bigC, wsb);
         if (s == RVSP_SUCCESS)
             s = rvsp_spgemm_wait(d);
     }
     if (s == RVSP_SUCCESS)
-        s = rvsp_hip_unpack_row_ptr(NULL, n, row_off, nrows, crp, ch.rp, bases);
+        s = rvsp_hip_unpack_row_ptr(BST, n, row_off, nrows, crp, ch.rp, bases);
     if (s == RVSP_SUCCESS)
     {
         int64_t out = 0;
@@ -223,14 +229,14 @@ static rvsp_status_t run_chunk(struct rvsp_spgemm_batch *batch, int32_t p0, int3
     if (s != RVSP_SUCCESS)
         free_chunk(&ch);
     rvsp_spgemm_descr_destroy(d);
-    rvsp_hip_free_async(arp, NULL);
-    rvsp_hip_free_async(aci, NULL);
-    rvsp_hip_free_async(ava, NULL);
-    rvsp_hip_free_async(brp, NULL);
-    rvsp_hip_free_async(bci, NULL);
-    rvsp_hip_free_async(bva, NULL);
-    rvsp_hip_free_async(crp, NULL);
-    rvsp_hip_stream_sync(NULL);
+    rvsp_hip_free_async(arp, BST);
+    rvsp_hip_free_async(aci, BST);
+    rvsp_hip_free_async(ava, BST);
+    rvsp_hip_free_async(brp, BST);
+    rvsp_hip_free_async(bci, BST);
+    rvsp_hip_free_async(bva, BST);
+    rvsp_hip_free_async(crp, BST);
+    rvsp_hip_stream_sync(BST);
     free(items);
     free(row_off);
     free(nrows);

@@ -59,6 +59,11 @@ rvsp_status_t device_ready()
         if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
             uint64_t thr = UINT64_MAX;
             (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+            // memory freed on one stream may serve allocations on another
+            int on = 1;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowOpportunistic, &on);
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowInternalDependencies, &on);
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolReuseFollowEventDependencies, &on);
         }
         g_pool_ready = true;
     }
@@ -263,6 +268,15 @@ void rvsp_hip_plan_destroy(rvsp_hip_plan_t *p)
 }
 
 void *rvsp_hip_plan_stream(const rvsp_hip_plan_t *p) { return p ? (void *)p->stream : nullptr; }
+
+void *rvsp_hip_oneshot_stream(void)
+{
+    static std::mutex mu;
+    static hipStream_t st = nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
+    return (void *)st;
+}
 
 const int32_t *rvsp_hip_plan_row_ptr(const rvsp_hip_plan_t *p) { return p ? p->crp : nullptr; }
 
