@@ -139,6 +139,34 @@ def cpu_baseline(A, B, target_s=10.0, seed=11):
 ALGOS = {"hip": rvsp.ALGO_HIP, "learned": rvsp.ALGO_HIP_LEARNED}
 
 
+def measured_traffic(workload):
+    """HBM bytes of one numeric pass, from the committed rocprofv3 FETCH_SIZE +
+    WRITE_SIZE summary of this workload (the line `numeric_hbm_bytes_per_pass: N`
+    of profiles/traffic_<workload>.md), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("numeric_hbm_bytes_per_pass:"):
+                return int(line.split(":")[1].strip())
+    return None
+
+
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB: a dispatch of known traffic
+    (mib MiB read, mib MiB written) that scales rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters in a profiled run."""
+    n = int(mib) << 20
+    src, dst = rvsp.DeviceBuffer(n), rvsp.DeviceBuffer(n)
+    rvsp.check("memset", rvsp.lib().rvsp_hip_memset(src.ptr, 1, n))
+    rvsp.check("sync", rvsp.lib().rvsp_hip_synchronize())
+    rvsp.check("copy", rvsp.lib().rvsp_hip_memcpy_d2d(dst.ptr, src.ptr, n))
+    rvsp.check("sync", rvsp.lib().rvsp_hip_synchronize())
+    src.free()
+    dst.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +175,9 @@ def main():
     ap.add_argument("--workload", default="rmat")
     ap.add_argument("--algo", default="learned", choices=["hip", "learned"],
                     help="learned: MLP row selectors (the plain heuristic is timed beside it)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one device copy of this many MiB (a known-size dispatch "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--selector-eval", action="store_true",
                     help="also time heuristic vs learned vs exhaustive-best numeric bins")
     ap.add_argument("--no-cpu", action="store_true")
@@ -158,6 +189,8 @@ def main():
         sys.exit(2)
     if not rvsp.gpu_available():
         raise SystemExit("no HIP device visible")
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
 
     if args.workload == "stencil":
         return bench_reuse(args)
@@ -201,9 +234,10 @@ def main():
         rvsp.set_timing(True)
         t0 = time.perf_counter()
         for i in range(args.steps):
+            ts = time.perf_counter()
             c = step(al)
             L.rvsp_csr_destroy(C.byref(c))
-            log(f"{tag} step {i + 1}/{args.steps}")
+            log(f"{tag} step {i + 1}/{args.steps}: {(time.perf_counter() - ts) * 1e3:.1f} ms")
         rvsp.check("sync", L.rvsp_hip_synchronize())
         t1 = time.perf_counter()
         rvsp.set_timing(False)
@@ -227,11 +261,7 @@ def main():
     achieved = alg_bytes / (num_avg_ms * 1e-3) / 1e9
     step_gbps = alg_bytes / (ms_step * 1e-3) / 1e9
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("numeric_hbm_bytes_per_pass")
+    traffic = measured_traffic(args.workload)
 
     cpu = None
     if not args.no_cpu:
@@ -363,7 +393,8 @@ def bench_reuse(args, n_sets=16):
         "graph_speedup": round(ms_call / ms_graph, 3)})
     rec["roofline"] = {"bound": "hbm", "kernel": "numeric pass (hipGraph replay)",
                        "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                       "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                       "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                       "traffic": measured_traffic(args.workload),
                        "algorithmic_bytes": alg_bytes, "numeric_ms": round(num_graph, 4)}
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline(A, A)
     print(json.dumps(rec), flush=True)
@@ -412,8 +443,9 @@ def bench_batched(args):
     rvsp.set_timing(True)
     t0 = time.perf_counter()
     for i in range(args.steps):
+        ts = time.perf_counter()
         chunks, _ = batched()
-        log(f"step {i + 1}/{args.steps}")
+        log(f"step {i + 1}/{args.steps}: {(time.perf_counter() - ts) * 1e3:.1f} ms")
     rvsp.check("sync", L.rvsp_hip_synchronize())
     ms_bat = (time.perf_counter() - t0) * 1e3 / args.steps
     rvsp.set_timing(False)
@@ -432,7 +464,8 @@ def bench_batched(args):
     rec["config"]["nnz_c"] = int(sum(nnzs))
     rec["roofline"] = {"bound": "hbm", "kernel": "numeric pass of every chunk (stacked problems)",
                        "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                       "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                       "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                       "traffic": measured_traffic(args.workload),
                        "algorithmic_bytes": alg_bytes, "numeric_ms": round(num_ms, 4)}
     rec["phases_ms_per_batch"] = {k: round(v[0] / max(args.steps, 1), 4) for k, v in ph.items()}
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline(*pairs[0])

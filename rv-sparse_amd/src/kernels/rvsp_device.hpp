@@ -44,6 +44,7 @@ constexpr int NB_COUNT = NB_WIN + 1;         // 62
 static_assert(NB_COUNT <= 64 && SB_COUNT <= 64, "bin tables hold 64 bins");
 
 constexpr int kBmsWords = 512;            // 16384 columns
+constexpr int kBmsSpillOps = 2 * kBmsWords;  // products above which a bitmap row spills
 constexpr int kBigPassWords = 16384;      // big-row bitmap: 2^19 columns per pass
 constexpr int kChunkCols = 1024;          // big-row chunk (window granularity)
 constexpr int kChunkWords = kChunkCols / 32;

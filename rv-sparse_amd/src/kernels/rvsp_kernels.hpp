@@ -55,6 +55,8 @@ struct SymArgs {
     int32_t *cci;
     BigArgs big;
     int *err;
+    const int64_t *ops;
+    uint32_t *bms_spill;  // kBmsWords words per one-pass-bitmap row (its slot in the bin), or null
 };
 
 struct NumArgs {

@@ -100,6 +100,18 @@ bool reuse_ranks_off()
     return off == 1;
 }
 
+// Count-pass bitmaps of one-pass-bitmap rows are kept for the fill unless
+// RVSP_BMS_SPILL=0 (read once).
+bool bms_spill_off()
+{
+    static int off = -1;
+    if (off < 0) {
+        const char *e = getenv("RVSP_BMS_SPILL");
+        off = e && e[0] == '0' ? 1 : 0;
+    }
+    return off == 1;
+}
+
 // Big-row numeric window shape: kBigShapes[0] unless RVSP_BIG_WINDOW names
 // another launchable one as "<outputs>x<chunks>" (read once).
 int big_window_shape()
@@ -186,6 +198,8 @@ struct rvsp_hip_plan {
     bool analysed = false;
 
     int2 *bcv = nullptr;  // B's (column, value) pairs, rewritten by every numeric pass
+    uint32_t *bms_spill = nullptr;  // count-pass bitmaps of the one-pass-bitmap rows
+    bool bms_spill_ok = false;      // the fill's bin lists are the count's (no recount)
 
     // reused structure: per-product output ranks of the LDS-path rows
     void *pmblock = nullptr;  // per-row product offsets (int64), scan scratch
@@ -212,6 +226,9 @@ static void plan_release(rvsp_hip_plan *p)
 {
     dfree(p->bcv, p->stream);
     p->bcv = nullptr;
+    dfree(p->bms_spill, p->stream);
+    p->bms_spill = nullptr;
+    p->bms_spill_ok = false;
     dfree(p->pmblock, p->stream);
     dfree(p->pmap, p->stream);
     p->pmblock = nullptr;
@@ -434,7 +451,17 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     }
 
     // ---- 2. exact nnz(C_i)
-    SymArgs sa{arp, aci, brp, bci, p->lo, p->hi, p->cnt, p->crp, nullptr, p->big, p->err};
+    if (p->sym_count[SB_BMS] > 0 && !bms_spill_off()) {
+        // count-pass bitmaps of the one-pass-bitmap rows, read back by the fill
+        s = dalloc((void **)&p->bms_spill, (size_t)p->sym_count[SB_BMS] * kBmsWords * 4, st);
+        if (s != RVSP_SUCCESS) {
+            plan_release(p);
+            return s;
+        }
+    }
+    p->bms_spill_ok = p->bms_spill != nullptr;
+    SymArgs sa{arp,    aci,    brp,    bci,    p->lo,  p->hi,       p->cnt,
+               p->crp, nullptr, p->big, p->err, p->ops, p->bms_spill};
     // rows without products get no kernel, so their count must start at 0
     HIP_TRY(hipMemsetAsync(p->cnt, 0, M * 4, st));
     const bool sized = sizer && p->hctr.moved > 0;
@@ -451,6 +478,8 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
         for (int b = 0; b < SB_COUNT; ++b) nfb += p->hctr.fb_hist[b];
         p->sized_rows = p->hctr.moved;
         p->fallback_rows = nfb;
+        // a recount rebuilds the bin lists: bitmap slots no longer match rows
+        if (nfb > 0) p->bms_spill_ok = false;
         if (nfb > 0) {
             int32_t fcount[SB_COUNT], foff[SB_COUNT];
             int32_t *fho = p->h_off[1];
@@ -529,7 +558,8 @@ rvsp_status_t rvsp_hip_plan_fill(rvsp_hip_plan_t *p, int32_t *cci)
 {
     if (!p || !cci) return RVSP_ERROR_NULL_POINTER;
     if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
-    SymArgs sa{p->arp, p->aci, p->brp, p->bci, p->lo, p->hi, p->cnt, p->crp, cci, p->big, p->err};
+    SymArgs sa{p->arp, p->aci, p->brp, p->bci, p->lo,  p->hi,  p->cnt,
+               p->crp, cci,    p->big, p->err, p->ops, p->bms_spill_ok ? p->bms_spill : nullptr};
     PhaseScope timer(p->stream, PH_FILL);
     launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off);
     HIP_TRY(hipGetLastError());

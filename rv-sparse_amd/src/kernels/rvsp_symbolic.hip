@@ -285,6 +285,8 @@ __global__ __launch_bounds__(256) void k_sym_bitmap(const int32_t *__restrict__ 
                                                     const int32_t *__restrict__ brp,
                                                     const int32_t *__restrict__ bci,
                                                     const int32_t *__restrict__ lo_col,
+                                                    const int64_t *__restrict__ ops,
+                                                    uint32_t *__restrict__ spill,
                                                     int32_t *__restrict__ cnt,
                                                     const int32_t *__restrict__ crp,
                                                     int32_t *__restrict__ cci)
@@ -301,9 +303,18 @@ __global__ __launch_bounds__(256) void k_sym_bitmap(const int32_t *__restrict__ 
     const int row = valid ? rows[ridx] : 0;
     uint32_t *Bm = bits + g * WORDS;
     const int32_t base = valid ? lo_col[row] : 0;
-    for (int w = lane; w < WORDS; w += G) Bm[w] = 0u;
+    // a row with many products per bitmap word keeps its count-pass bitmap
+    // in HBM (slot ridx of the bin's list, the same in both passes), so the
+    // fill pass reads it back instead of walking the products again
+    const bool spilled = spill && valid && ops[row] >= kBmsSpillOps;
+    uint32_t *slot = spilled ? spill + ridx * WORDS : nullptr;
+    if (FILL && spilled) {
+        for (int w = lane; w < WORDS; w += G) Bm[w] = slot[w];
+    } else {
+        for (int w = lane; w < WORDS; w += G) Bm[w] = 0u;
+    }
     __syncthreads();
-    if (valid) {
+    if (valid && !(FILL && spilled)) {
         const int as = arp[row], ae = arp[row + 1];
         for (int ap = as; ap < ae; ++ap) {
             const int k = aci[ap];
@@ -315,6 +326,8 @@ __global__ __launch_bounds__(256) void k_sym_bitmap(const int32_t *__restrict__ 
         }
     }
     __syncthreads();
+    if (!FILL && spilled)
+        for (int w = lane; w < WORDS; w += G) slot[w] = Bm[w];
     // ordered scan: thread `lane` owns words [lane*WPT, (lane+1)*WPT)
     int local = 0;
 #pragma unroll 4
@@ -544,7 +557,8 @@ static void launch_sym_bins(hipStream_t st, const SymArgs &a, const int32_t *per
         case SB_BMS: {
             const int grid = (nr + 3) / 4;
             k_sym_bitmap<64, kBmsWords, FILL><<<grid, 256, 0, st>>>(
-                rows, nr, a.arp, a.aci, a.brp, a.bci, a.lo, a.cnt, a.crp, a.cci);
+                rows, nr, a.arp, a.aci, a.brp, a.bci, a.lo, a.ops, a.bms_spill, a.cnt, a.crp,
+                a.cci);
             break;
         }
         case SB_BIG:
