@@ -4,8 +4,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
 #include <stdio.h>
@@ -70,6 +72,39 @@ rvsp_status_t device_ready()
     return RVSP_SUCCESS;
 }
 
+// Blocks freed on the library's one-shot stream, kept for later calls on
+// that stream. Stream order makes the reuse safe without events: every
+// earlier use of a block was queued on the same stream before its free. The
+// stream-ordered pool alone does not keep them: at a stream synchronisation
+// the runtime hands most freed blocks back to the OS whatever the release
+// threshold, and re-mapping C's arrays and the big-row scratch each call
+// costs about a millisecond per few GB, with occasional 50-150 ms stalls.
+struct BlockCache {
+    std::mutex mu;
+    std::multimap<size_t, void *> idle;        // size -> idle block
+    std::unordered_map<void *, size_t> sizes;  // every block the cache handed out
+    size_t idle_bytes = 0;
+    size_t limit = 0;  // idle bytes kept at most: a quarter of device memory
+};
+BlockCache g_cache;
+
+size_t cache_round(size_t bytes)
+{
+    const size_t g = bytes >= ((size_t)1 << 24) ? ((size_t)2 << 20) : (size_t)4096;
+    return (bytes + g - 1) / g * g;
+}
+
+// hands every idle block back to the pool (caller holds the lock)
+void cache_flush_locked(hipStream_t st)
+{
+    for (auto &e : g_cache.idle) {
+        g_cache.sizes.erase(e.second);
+        (void)hipFreeAsync(e.second, st);
+    }
+    g_cache.idle.clear();
+    g_cache.idle_bytes = 0;
+}
+
 rvsp_status_t dalloc(void **p, size_t bytes, hipStream_t st)
 {
     *p = nullptr;
@@ -78,13 +113,64 @@ rvsp_status_t dalloc(void **p, size_t bytes, hipStream_t st)
         return RVSP_ERROR_ALLOCATION_FAILED;
     }
     if (bytes == 0) bytes = 256;
-    HIP_TRY(hipMallocAsync(p, bytes, st));
+    if (st != (hipStream_t)rvsp_hip_oneshot_stream()) {
+        HIP_TRY(hipMallocAsync(p, bytes, st));
+        return RVSP_SUCCESS;
+    }
+    bytes = cache_round(bytes);
+    std::lock_guard<std::mutex> lk(g_cache.mu);
+    if (g_cache.limit == 0) {
+        size_t fr = 0, tot = 0;
+        g_cache.limit = hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 4 : ((size_t)1 << 32);
+    }
+    // best fit, wasting at most an eighth of the block
+    auto it = g_cache.idle.lower_bound(bytes);
+    if (it != g_cache.idle.end() && it->first - bytes <= it->first / 8) {
+        *p = it->second;
+        g_cache.idle_bytes -= it->first;
+        g_cache.idle.erase(it);
+        return RVSP_SUCCESS;
+    }
+    hipError_t e = hipMallocAsync(p, bytes, st);
+    if (e == hipErrorOutOfMemory && !g_cache.idle.empty()) {
+        (void)hipGetLastError();  // recovered below: not a launch error
+        cache_flush_locked(st);
+        (void)hipStreamSynchronize(st);
+        e = hipMallocAsync(p, bytes, st);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return map_err(e);
+    }
+    g_cache.sizes[*p] = bytes;
     return RVSP_SUCCESS;
 }
 
 void dfree(void *p, hipStream_t st)
 {
-    if (p) (void)hipFreeAsync(p, st);
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(g_cache.mu);
+        auto it = g_cache.sizes.find(p);
+        if (it != g_cache.sizes.end()) {
+            if (st == (hipStream_t)rvsp_hip_oneshot_stream()) {
+                g_cache.idle.emplace(it->second, p);
+                g_cache.idle_bytes += it->second;
+                // over the limit: the smallest idle blocks go back first
+                while (g_cache.idle_bytes > g_cache.limit && !g_cache.idle.empty()) {
+                    auto v = g_cache.idle.begin();
+                    g_cache.idle_bytes -= v->first;
+                    g_cache.sizes.erase(v->second);
+                    (void)hipFreeAsync(v->second, st);
+                    g_cache.idle.erase(v);
+                }
+                return;
+            }
+            g_cache.sizes.erase(it);  // freed on another stream: the pool takes it
+        }
+    }
+    (void)hipFreeAsync(p, st);
 }
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -810,7 +896,7 @@ rvsp_status_t rvsp_hip_alloc_async(void **ptr, size_t bytes, void *stream)
 
 rvsp_status_t rvsp_hip_free_async(void *ptr, void *stream)
 {
-    if (ptr) HIP_TRY(hipFreeAsync(ptr, (hipStream_t)stream));
+    dfree(ptr, (hipStream_t)stream);
     return RVSP_SUCCESS;
 }
 

@@ -276,6 +276,45 @@ def test_device_oneshot_matches_host_oneshot():
     check_product(A, B, C_, "device-oneshot")
 
 
+def test_device_oneshots_reuse_freed_blocks_without_aliasing_live_ones():
+    """Device one-shots recycle blocks freed on the library stream: a C that
+    is still alive is never handed out again, and results stay exact when a
+    product lands in blocks another product used."""
+    L = rvsp.lib()
+    prods = [W.SMALL["mixed"](), W.SMALL["rmat"](), W.SMALL["uniform"]()]
+    dev = [(rvsp.DeviceCsr(A), rvsp.DeviceCsr(B)) for A, B in prods]
+
+    def run(i):
+        a, b = dev[i][0].struct(), dev[i][1].struct()
+        c = rvsp.CsrStruct()
+        rvsp.check("csr_device", L.rvsp_spgemm_csr_device_algo(C.byref(a), C.byref(b), C.byref(c),
+                                                               rvsp.ALGO_HIP))
+        return c
+
+    def spans(c):
+        return [(c.row_ptr, c.row_ptr + 4 * (c.rows + 1)), (c.col_idx, c.col_idx + 4 * max(c.nnz, 1)),
+                (c.values, c.values + 4 * max(c.nnz, 1))]
+
+    def check(i, c, tag):
+        A, B = prods[i]
+        check_product(A, B, rvsp.device_csr_to_host(c), tag)
+
+    for rnd in range(2):
+        say(f"block reuse round {rnd}")
+        live = [run(i) for i in range(3)]
+        ranges = [r for c in live for r in spans(c)]
+        for x in range(len(ranges)):
+            for y in range(x + 1, len(ranges)):
+                assert ranges[x][1] <= ranges[y][0] or ranges[y][1] <= ranges[x][0], "live blocks overlap"
+        for i, c in enumerate(live):
+            check(i, c, f"round {rnd} product {i}")
+        L.rvsp_csr_destroy(C.byref(live[1]))
+        again = run(0)  # may take the freed blocks of the other product
+        check(0, again, f"round {rnd} product 0 in recycled blocks")
+        for c in (live[0], live[2], again):
+            L.rvsp_csr_destroy(C.byref(c))
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_device_descriptor_reuse_with_redrawn_values(graph):
     say(f"device reuse graph={graph}")
