@@ -13,7 +13,9 @@
 //            prefix (a rank dictionary: column -> position in C_i), zeroes a
 //            dense accumulator over the window's OUTPUT positions, walks the
 //            row's A entries in stored order with one cursor per entry
-//            (carried from window to window), and adds each product into
+//            (carried from window to window; the first cursor of each group
+//            of 64 also holds the group's smallest next column, so groups
+//            with nothing in the window are skipped unread), and adds each product into
 //            acc[rank(col)] with one LDS float add. One wave, entries in
 //            ascending k, columns of one B row distinct: the same ordered,
 //            separately rounded accumulation as rv-sparse's scalar kernel
@@ -452,7 +454,9 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
         for (int u = 0; u < 4; ++u) {
             const int ap = base + u * 64 + lane;
             const int32_t nc = ap < ae && lo4[u] < e4[u] ? a.bci[lo4[u]] : INT32_MAX;
-            if (ap < ae) cur_a[ap - as] = make_int4(lo4[u], e4[u], nc, 0);
+            // the group's first cursor also holds the group's smallest column
+            const int gmin = wave_min(nc);
+            if (ap < ae) cur_a[ap - as] = make_int4(lo4[u], e4[u], nc, lane == 0 ? gmin : 0);
         }
     }
 
@@ -493,36 +497,46 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
         for (int i = lane; i < outs; i += 64) acc[i] = 0.0f;
         wave_sync();
 
-        for (int base = as; base < ae; base += 64) {
-            const int ap = base + lane;
-            int4 ce = make_int4(0, 0, INT32_MAX, 0);
-            float av = 0.0f;
-            if (ap < ae) {
-                ce = cur_a[ap - as];
-                av = a.aval[ap];
-            }
-            // the cursor carries the column it points at: no load to test it
-            const bool live = ce.x < ce.y && ce.z < c1;
-            int cur = ce.x, nxt = ce.z;
-            uint64_t mask = __ballot(live);
-            while (mask) {
-                // up to four live entries: issue all their loads first
-                PieceGroup g0;
-                g0.take(mask, ce, av, a);
+        // groups of 64 entries whose smallest next column lies before the
+        // window's end, found 64 groups at a time; the others are skipped
+        for (int gbase = as; gbase < ae; gbase += 64 * 64) {
+            const int gp = gbase + lane * 64;
+            const int gmin0 = gp < ae ? cur_a[gp - as].w : INT32_MAX;
+            uint64_t groups = __ballot(gmin0 < c1);
+            while (groups) {
+                const int base = gbase + 64 * (__ffsll((unsigned long long)groups) - 1);
+                groups &= groups - 1;
+                const int ap = base + lane;
+                int4 ce = make_int4(0, 0, INT32_MAX, 0);
+                float av = 0.0f;
+                if (ap < ae) {
+                    ce = cur_a[ap - as];
+                    av = a.aval[ap];
+                }
+                // the cursor carries the column it points at: no load to test it
+                const bool live = ce.x < ce.y && ce.z < c1;
+                int cur = ce.x, nxt = ce.z;
+                uint64_t mask = __ballot(live);
+                while (mask) {
+                    // up to four live entries: issue all their loads first
+                    PieceGroup g0;
+                    g0.take(mask, ce, av, a);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (u < g0.n) {
-                        const int2 nc = big_piece(g0.c[u], g0.e[u], g0.av[u], g0.col[u],
-                                                  g0.bv[u], c0, c1, outs, W, P, acc, a);
-                        compiler_fence();
-                        if (lane == g0.j[u]) {
-                            cur = nc.x;
-                            nxt = nc.y;
+                    for (int u = 0; u < 4; ++u) {
+                        if (u < g0.n) {
+                            const int2 nc = big_piece(g0.c[u], g0.e[u], g0.av[u], g0.col[u],
+                                                      g0.bv[u], c0, c1, outs, W, P, acc, a);
+                            compiler_fence();
+                            if (lane == g0.j[u]) {
+                                cur = nc.x;
+                                nxt = nc.y;
+                            }
                         }
                     }
                 }
+                const int gmin = wave_min(nxt);  // lanes past the row hold INT32_MAX
+                if (live || lane == 0) cur_a[ap - as] = make_int4(cur, ce.y, nxt, lane == 0 ? gmin : 0);
             }
-            if (live) cur_a[ap - as] = make_int4(cur, ce.y, nxt, 0);
         }
         wave_sync();
         for (int i = lane; i < outs; i += 64) a.cval[rowbase + o0 + i] = acc[i];

@@ -161,6 +161,13 @@ __device__ __forceinline__ int group_sum(int v)
     return v;
 }
 
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
 // Exclusive scan over a 256-thread block (4 waves); *total gets the sum.
 // Every thread of the block must call it.
 __device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t *total)
