@@ -20,6 +20,8 @@
 //            ascending k, columns of one B row distinct: the same ordered,
 //            separately rounded accumulation as rv-sparse's scalar kernel
 //            (gustavson_core_f32.inc:31-63, accum_scalar_f32.c:67-70).
+#include <stdlib.h>
+
 #include "rvsp_device.hpp"
 #include "rvsp_kernels.hpp"
 
@@ -51,9 +53,8 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
     constexpr int NWV = NT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int row = rows[blockIdx.x];
-    const int32_t rlo = g.lo[row], rhi = g.hi[row];
-    const int nchunks = (int)(((int64_t)rhi - rlo + kChunkCols) / kChunkCols);
-    const int nwords = nchunks * kChunkWords;
+    const int32_t blo = big_base(g.lo[row]);
+    const int nwords = big_nchunks(g.lo[row], g.hi[row]) * kChunkWords;
     const int passw = g.pass_words;
     if (tid == 0) {
         s_off = (long long)atomicAdd(g.bits_fill, (unsigned long long)nwords);
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
     int total = 0;
     for (int pw0 = 0; pw0 < nwords; pw0 += passw) {
         const int pwn = min(passw, nwords - pw0);
-        const int32_t base_col = rlo + pw0 * 32;
+        const int32_t base_col = blo + pw0 * 32;
         const int32_t pcols = pwn * 32;
         for (int w = tid; w < pwn; w += NT) bits[w] = 0u;
         __syncthreads();
@@ -79,8 +80,8 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
             }
             const bool lng = mbe - mbs > kSymLong;
             int64_t npieces;
-            const int pfx =
-                (int)block_excl_scan(lng ? (mbe - mbs + kSymPiece - 1) / kSymPiece : 0, &npieces);
+            const int pfx = (int)block_excl_scan<NWV>(
+                lng ? (mbe - mbs + kSymPiece - 1) / kSymPiece : 0, &npieces);
             s_pfx[tid] = pfx;
             s_bs[tid] = mbs;
             s_be[tid] = mbe;
@@ -142,60 +143,70 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
 }
 
 // ---------------------------------------------------------------------------
-// Structure fill of big rows from their bitmaps: one block per row, tiles
-// of 1024 words (4 per thread). A tile's set bits are expanded into an LDS
-// staging buffer at their ordered positions, then copied out contiguously,
-// so the global writes are coalesced; a tile with more than kFillStage
-// columns (a nearly dense span) writes straight from the threads instead.
+// Structure fill of big rows from their bitmaps: one wave per segment of the
+// numeric plan, window by window (at most PW outputs, MAXCH chunks). The
+// wave loads the window's bitmap words 64 at a time (one per lane), ranks
+// each round with a wave scan, expands the set bits into an LDS stage at
+// their ordered positions and copies the stage out in coalesced runs.
 // ---------------------------------------------------------------------------
-constexpr int kFillPer = 4;
-constexpr int kFillTile = 256 * kFillPer;  // words
-constexpr int kFillStage = 8192;           // staged columns per tile
-
-__global__ __launch_bounds__(256) void k_big_fill(const int32_t *__restrict__ rows, int32_t nrows,
-                                                  BigArgs g, const int32_t *__restrict__ crp,
-                                                  int32_t *__restrict__ cci)
+template <int NWV, int PW, int MAXCH>
+__global__ __launch_bounds__(NWV * 64) void k_big_fill(int32_t nsegs, BigArgs g,
+                                                       const int32_t *__restrict__ crp,
+                                                       int32_t *__restrict__ cci)
 {
-    __shared__ int32_t stage[kFillStage];
-    const int row = rows[blockIdx.x];
-    const int32_t rlo = g.lo[row], rhi = g.hi[row];
-    const int nwords = (int)(((int64_t)rhi - rlo + kChunkCols) / kChunkCols) * kChunkWords;
-    const int64_t off = g.big_off[row];
-    int out = crp[row];
-    for (int t0 = 0; t0 < nwords; t0 += kFillTile) {
-        uint32_t w[kFillPer];
-        int local = 0;
-        const int wbase = t0 + threadIdx.x * kFillPer;
+    constexpr int PER = MAXCH * kChunkWords / 64;
+    __shared__ int32_t stage_s[NWV][PW];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t it = (int64_t)blockIdx.x * NWV + wv;
+    if (it >= nsegs) return;  // whole wave leaves; no block barriers below
+    int32_t *stage = stage_s[wv];
+    const int4 sg = g.segs[it];
+    const int row = sg.x, w0 = sg.y, w1 = sg.z;
+    const int32_t blo = big_base(g.lo[row]);
+    const int nchunks = big_nchunks(g.lo[row], g.hi[row]);
+    const int64_t boff = g.big_off[row];
+    const int64_t c32 = boff / kChunkWords;
+    const int nwin = g.big_nwin[row];
+    const int rowbase = crp[row];
+    const int nz = crp[row + 1] - rowbase;
+    for (int w = w0; w < w1; ++w) {
+        const int ch0 = g.win_chunk0[c32 + w];
+        const int ch1 = w + 1 < nwin ? g.win_chunk0[c32 + w + 1] : nchunks;
+        const int o0 = g.win_out[c32 + w];
+        const int outs = (w + 1 < nwin ? g.win_out[c32 + w + 1] : nz) - o0;
+        const int nw = (ch1 - ch0) * kChunkWords;  // <= MAXCH * kChunkWords
+        // round r: lane l holds word r * 64 + l (coalesced), so no lane
+        // expands more than one word's 32 bits per round
+        uint32_t wl[PER];
 #pragma unroll
-        for (int j = 0; j < kFillPer; ++j) {
-            w[j] = wbase + j < nwords ? g.gbits[off + wbase + j] : 0u;
-            local += __popc(w[j]);
+        for (int r = 0; r < PER; ++r) {
+            const int i = r * 64 + lane;
+            wl[r] = i < nw ? g.gbits[boff + (int64_t)ch0 * kChunkWords + i] : 0u;
         }
-        int64_t tot64;
-        const int rel = (int)block_excl_scan(local, &tot64);
-        const int tot = (int)tot64;
-        const bool staged = tot <= kFillStage;
-        int pos = rel;
+        const int32_t cb0 = blo + ch0 * kChunkCols + lane * 32;
+        int run = 0;
 #pragma unroll
-        for (int j = 0; j < kFillPer; ++j) {
-            uint32_t x = w[j];
-            const int32_t cb = rlo + (wbase + j) * 32;
+        for (int r = 0; r < PER; ++r) {
+            const int cnt = __popc(wl[r]);
+            int incl = cnt;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            int pos = run + incl - cnt;
+            uint32_t x = wl[r];
             while (x) {
                 const int b = __ffs(x) - 1;
                 x &= x - 1;
-                if (staged)
-                    stage[pos] = cb + b;
-                else
-                    cci[out + pos] = cb + b;
+                if (pos < PW) stage[pos] = cb0 + r * 64 * 32 + b;
                 ++pos;
             }
+            run += lane_bcast(incl, 63);
         }
-        if (staged) {  // block-uniform
-            __syncthreads();
-            for (int i = threadIdx.x; i < tot; i += 256) cci[out + i] = stage[i];
-            __syncthreads();
-        }
-        out += tot;
+        wave_sync();
+        for (int i = lane; i < outs && i < PW; i += 64) cci[rowbase + o0 + i] = stage[i];
+        wave_sync();
     }
 }
 
@@ -216,8 +227,7 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool valid = j < nrows;    // every thread joins the block scans below
     const int row = valid ? rows[j] : 0;
-    const int32_t rlo = g.lo[row], rhi = g.hi[row];
-    const int nchunks = (int)(((int64_t)rhi - rlo + kChunkCols) / kChunkCols);
+    const int nchunks = big_nchunks(g.lo[row], g.hi[row]);
     const int64_t c32 = valid ? g.big_off[row] / kChunkWords : 0;
     const int alen = g.arp[row + 1] - g.arp[row];
     const int nz = crp[row + 1] - crp[row];
@@ -330,7 +340,8 @@ constexpr int kPieceBatches = 4;
 
 __device__ __forceinline__ int2 big_piece(int cj, int ej, float aj, int32_t col, float bv,
                                           int32_t c0, int32_t c1, int outs, const uint32_t *W,
-                                          const uint16_t *P, float *acc, const NumArgs &a)
+                                          const uint16_t *P, float *acc, const NumArgs &a,
+                                          const int2 *__restrict__ pairs)
 {
     const int lane = threadIdx.x & 63;
     int nin = big_batch(aj, col, bv, c0, c1, outs, W, P, acc, a);
@@ -342,7 +353,7 @@ __device__ __forceinline__ int2 big_piece(int cj, int ej, float aj, int32_t col,
 #pragma unroll
         for (int b = 0; b < kPieceBatches; ++b) {
             const int bp = cj + b * 64 + lane;
-            const int2 cv = bp < ej ? a.bcv[bp] : make_int2(INT32_MAX, 0);
+            const int2 cv = bp < ej ? pairs[bp] : make_int2(INT32_MAX, 0);
             cb[b] = cv.x;
             vb[b] = __int_as_float(cv.y);
         }
@@ -360,14 +371,16 @@ __device__ __forceinline__ int2 big_piece(int cj, int ej, float aj, int32_t col,
 }
 
 // Up to four live A entries of a window, taken from a lane mask in lane
-// (= ascending k) order, with the first 64 entries of each piece loaded.
+// (= ascending k) order, with the first 64 entries of each piece loaded
+// from `pairs` (B's pairs, or their column-blocked copy).
 struct PieceGroup {
     int n;
     int j[4], c[4], e[4];
     float av[4], bv[4];
     int32_t col[4];
 
-    __device__ __forceinline__ void take(uint64_t &mask, int4 ce, float a_val, const NumArgs &a)
+    __device__ __forceinline__ void take(uint64_t &mask, int4 ce, float a_val,
+                                         const int2 *__restrict__ pairs)
     {
         const int lane = threadIdx.x & 63;
         n = 0;
@@ -387,7 +400,7 @@ struct PieceGroup {
             av[u] = lane_bcast(a_val, j[u]);
             const int bp = c[u] + lane;
             const bool ok = u < n && bp < e[u];
-            const int2 cv = ok ? a.bcv[bp] : make_int2(INT32_MAX, 0);
+            const int2 cv = ok ? pairs[bp] : make_int2(INT32_MAX, 0);
             col[u] = cv.x;
             bv[u] = __int_as_float(cv.y);
         }
@@ -411,8 +424,9 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
     const int row = sg.x, w0 = sg.y, w1 = sg.z;
     int4 *cur_a = g.cursors + g.seg_cur[it];
     const int as = a.arp[row], ae = a.arp[row + 1];
-    const int32_t rlo = g.lo[row], rhi = g.hi[row];
-    const int nchunks = (int)(((int64_t)rhi - rlo + kChunkCols) / kChunkCols);
+    const int32_t rlo = big_base(g.lo[row]);
+    const int nchunks = big_nchunks(g.lo[row], g.hi[row]);
+    const int2 *__restrict__ pairs = a.bcv;
     const int64_t boff = g.big_off[row];
     const int64_t c32 = boff / kChunkWords;
     const int nwin = g.big_nwin[row];
@@ -520,12 +534,13 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
                 while (mask) {
                     // up to four live entries: issue all their loads first
                     PieceGroup g0;
-                    g0.take(mask, ce, av, a);
+                    g0.take(mask, ce, av, pairs);
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         if (u < g0.n) {
                             const int2 nc = big_piece(g0.c[u], g0.e[u], g0.av[u], g0.col[u],
-                                                      g0.bv[u], c0, c1, outs, W, P, acc, a);
+                                                      g0.bv[u], c0, c1, outs, W, P, acc, a,
+                                                      pairs);
                             compiler_fence();
                             if (lane == g0.j[u]) {
                                 cur = nc.x;
@@ -547,23 +562,53 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-void launch_sym_big(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g)
+template <int NT>
+static void launch_sym_big_nt(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g)
 {
-    if (nrows <= 0) return;
-    const size_t lds = (size_t)g.pass_words * 4;
     static bool raised = false;  // allow the largest pass once per process
     if (!raised) {
-        (void)hipFuncSetAttribute((const void *)&k_sym_big<256>,
+        (void)hipFuncSetAttribute((const void *)&k_sym_big<NT>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, kBigPassWords * 4);
         raised = true;
     }
-    k_sym_big<256><<<nrows, 256, lds, st>>>(rows, nrows, g);
+    k_sym_big<NT><<<nrows, NT, (size_t)g.pass_words * 4, st>>>(rows, nrows, g);
 }
 
-void launch_big_fill(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,
-                     const int32_t *crp, int32_t *cci)
+void launch_sym_big(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g)
 {
-    if (nrows > 0) k_big_fill<<<nrows, 256, 0, st>>>(rows, nrows, g, crp, cci);
+    if (nrows <= 0) return;
+    // 512 threads (measured fastest of 256 / 512 / 1024 on R-MAT 2^18 and the
+    // mixed workload); RVSP_SYM_BIG_THREADS picks another for tuning
+    const char *e = getenv("RVSP_SYM_BIG_THREADS");
+    const int nt = e ? atoi(e) : 512;
+    if (nt == 1024)
+        launch_sym_big_nt<1024>(st, rows, nrows, g);
+    else if (nt == 256)
+        launch_sym_big_nt<256>(st, rows, nrows, g);
+    else
+        launch_sym_big_nt<512>(st, rows, nrows, g);
+}
+
+template <int PW, int MAXCH>
+static void launch_big_fill_shape(hipStream_t st, const BigArgs &g, const int32_t *crp,
+                                  int32_t *cci)
+{
+    constexpr int NWV = 4;
+    k_big_fill<NWV, PW, MAXCH><<<(g.nsegs + NWV - 1) / NWV, NWV * 64, 0, st>>>(g.nsegs, g, crp, cci);
+}
+
+void launch_big_fill(hipStream_t st, const BigArgs &g, const int32_t *crp, int32_t *cci)
+{
+    if (g.nsegs <= 0) return;
+    // the shape must be one of kBigShapes (the plan only sets those)
+    if (g.win_outs == 4096 && g.win_chunks == 32)
+        launch_big_fill_shape<4096, 32>(st, g, crp, cci);
+    else if (g.win_chunks == 32)
+        launch_big_fill_shape<2048, 32>(st, g, crp, cci);
+    else if (g.win_outs == 1024 && g.win_chunks == 8)
+        launch_big_fill_shape<1024, 8>(st, g, crp, cci);
+    else  // 2048 x 16 and 1024 x 16 (a 1024-column chunk never exceeds 2048 outputs)
+        launch_big_fill_shape<2048, 16>(st, g, crp, cci);
 }
 
 void launch_big_plan(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,

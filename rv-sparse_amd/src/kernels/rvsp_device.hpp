@@ -58,6 +58,14 @@ constexpr int kSymHashMaxU = 2048;        // symbolic hash rows: min(ops,span) <
 
 __device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
 
+// A big row's bitmap starts at its lowest product column rounded down to a
+// chunk, so its chunks are aligned to global 1024-column boundaries.
+__host__ __device__ __forceinline__ int32_t big_base(int32_t lo) { return lo & ~(kChunkCols - 1); }
+__host__ __device__ __forceinline__ int big_nchunks(int32_t lo, int32_t hi)
+{
+    return (int)(((int64_t)hi - big_base(lo)) / kChunkCols) + 1;
+}
+
 // One A entry of a row walk, as the lanes of a group share it: its value,
 // its B row [bs, be), and (rank walks) the products of earlier entries of
 // its chunk. Read by every lane of the group with one 16-byte LDS load.
@@ -168,11 +176,12 @@ __device__ __forceinline__ int wave_min(int v)
     return v;
 }
 
-// Exclusive scan over a 256-thread block (4 waves); *total gets the sum.
-// Every thread of the block must call it.
+// Exclusive scan over a block of NWV waves (256 threads by default); *total
+// gets the sum. Every thread of the block must call it.
+template <int NWV = 4>
 __device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t *total)
 {
-    __shared__ int64_t wsum[4];
+    __shared__ int64_t wsum[NWV];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     int64_t x = v;
 #pragma unroll
@@ -184,7 +193,7 @@ __device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t *total)
     __syncthreads();
     int64_t before = 0, all = 0;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
+    for (int w = 0; w < NWV; ++w) {
         if (w < wid) before += wsum[w];
         all += wsum[w];
     }

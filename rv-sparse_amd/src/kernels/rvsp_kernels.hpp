@@ -37,10 +37,12 @@ struct BigArgs {
     int4 *segs;           // (row, first window, end window, 0)
     int64_t *seg_cur;     // per segment: offset of its cursor block
     int4 *cursors;        // per (segment, A entry): (next B position, end of B row,
-                          // column at that position or INT32_MAX, 0)
+                          // column at that position or INT32_MAX, 0); the first cursor
+                          // of a group of 64 keeps the group's smallest column in .w
     int pass_words;       // symbolic bitmap words per pass (LDS), multiple of kChunkWords
     int win_outs;         // numeric window: at most this many outputs ...
     int win_chunks;       // ... and this many chunks (one of the launchable shapes)
+    int32_t nsegs;        // segments planned (set once the analysis has read the count)
 };
 
 // Launchable big-row window shapes (outputs, chunks); the first is the default.
@@ -99,8 +101,8 @@ void launch_interleave(hipStream_t st, int32_t nnz, const int32_t *bci, const fl
 // One hash or dense numeric bin over the given rows (false: not such a bin).
 bool launch_num_bin(hipStream_t st, const NumArgs &a, int bin, const int32_t *rows, int nrows);
 void launch_sym_big(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g);
-void launch_big_fill(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,
-                     const int32_t *crp, int32_t *cci);
+// Big-row structure fill over every planned segment (g.nsegs).
+void launch_big_fill(hipStream_t st, const BigArgs &g, const int32_t *crp, int32_t *cci);
 void launch_big_plan(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,
                      const int32_t *crp, AnalysisCounters *ctr);
 void launch_num_big(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g);

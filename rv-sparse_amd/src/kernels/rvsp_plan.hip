@@ -199,18 +199,15 @@ bool bms_spill_off()
 }
 
 // Big-row numeric window shape: kBigShapes[0] unless RVSP_BIG_WINDOW names
-// another launchable one as "<outputs>x<chunks>" (read once).
+// another launchable one as "<outputs>x<chunks>" (read at every analysis).
 int big_window_shape()
 {
-    static int shape = -1;
-    if (shape < 0) {
-        shape = 0;
-        const char *e = getenv("RVSP_BIG_WINDOW");
-        int o = 0, c = 0;
-        if (e && sscanf(e, "%dx%d", &o, &c) == 2)
-            for (int i = 0; i < kBigShapeCount; ++i)
-                if (kBigShapes[i][0] == o && kBigShapes[i][1] == c) shape = i;
-    }
+    int shape = 0;
+    const char *e = getenv("RVSP_BIG_WINDOW");
+    int o = 0, c = 0;
+    if (e && sscanf(e, "%dx%d", &o, &c) == 2)
+        for (int i = 0; i < kBigShapeCount; ++i)
+            if (kBigShapes[i][0] == o && kBigShapes[i][1] == c) shape = i;
     return shape;
 }
 
@@ -602,6 +599,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     HIP_TRY(hipStreamSynchronize(st));
     p->nnzc = p->hctr.nnz_total;
     p->nseg = p->hctr.nseg;
+    p->big.nsegs = p->nseg;
     if (p->nseg > 0) {
         s = dalloc((void **)&p->cursors, (size_t)p->hctr.cur_total * sizeof(int4), st);
         if (s != RVSP_SUCCESS) {

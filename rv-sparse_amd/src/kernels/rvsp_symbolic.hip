@@ -77,8 +77,7 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
         sbin_out[row] = (uint8_t)b;
         atomicAdd(&hist[b], 1);
         if (b == SB_BIG) {
-            const int nchunks = (int)(((int64_t)hi - lo + kChunkCols) / kChunkCols);
-            atomicAdd(&bcap, (unsigned long long)nchunks * kChunkWords);
+            atomicAdd(&bcap, (unsigned long long)big_nchunks(lo, hi) * kChunkWords);
         }
         if (ops) atomicAdd(&bprod, (unsigned long long)ops);
     }
@@ -563,7 +562,7 @@ static void launch_sym_bins(hipStream_t st, const SymArgs &a, const int32_t *per
         }
         case SB_BIG:
             if constexpr (FILL)
-                launch_big_fill(st, rows, nr, a.big, a.crp, a.cci);
+                launch_big_fill(st, a.big, a.crp, a.cci);
             else
                 launch_sym_big(st, rows, nr, a.big);
             break;

@@ -220,6 +220,23 @@ def test_wide_random_rows_windows(alen):
     check_product(A, B, rvsp.spgemm_raw(A, B), f"wide-{alen}")
 
 
+@pytest.mark.parametrize("threads", ["256", "512", "1024"])
+@pytest.mark.parametrize("shape", ["2048x16", "4096x32", "2048x32", "1024x16", "1024x8"])
+def test_big_rows_every_window_shape_and_count_block(shape, threads, monkeypatch):
+    """Big rows under every launchable numeric window shape and every block
+    size of the symbolic count: windows that split rows at many places
+    (segments starting mid-row, pieces ending inside windows, groups of
+    entries with nothing in a window)."""
+    monkeypatch.setenv("RVSP_BIG_WINDOW", shape)
+    monkeypatch.setenv("RVSP_SYM_BIG_THREADS", threads)
+    say(f"big rows shape={shape} count threads={threads}")
+    A, B = W.rmat(scale=15, seed=31)
+    check_product(A, B, rvsp.spgemm_raw(A, B), f"rmat-15 {shape} threads={threads}")
+    B2 = rvsp.gen_random_rows(3000, 100000, 250, 350, 77)
+    A2 = rvsp.gen_random_rows(40, 3000, 700, 700, 78)
+    check_product(A2, B2, rvsp.spgemm_raw(A2, B2), f"wide-700 {shape} threads={threads}")
+
+
 # ---------------------------------------------------------------------------
 # edge cases
 # ---------------------------------------------------------------------------
