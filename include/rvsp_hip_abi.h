@@ -66,6 +66,16 @@ rvsp_status_t rvsp_hip_plan_numeric(rvsp_hip_plan_t *plan,
                                     const int32_t *c_col_idx,
                                     float *c_values);
 
+/* Structure fill and numeric pass together, for a C whose column indices
+ * are not yet written: the big rows' columns come out of the numeric pass,
+ * which already holds each window's bitmap, instead of a separate fill.
+ * Queued on the plan stream; same results as fill then numeric. */
+rvsp_status_t rvsp_hip_plan_fill_numeric(rvsp_hip_plan_t *plan,
+                                         const float *a_values,
+                                         const float *b_values,
+                                         int32_t *c_col_idx,
+                                         float *c_values);
+
 /* Same numeric pass, captured once into a hipGraph and replayed while the
  * four pointers stay the same (re-captured when they change). */
 rvsp_status_t rvsp_hip_plan_numeric_graph(rvsp_hip_plan_t *plan,

@@ -142,6 +142,7 @@ def _declare(lib):
         "rvsp_hip_plan_row_ptr": (V, [V]),
         "rvsp_hip_plan_fill": (C.c_int, [V, V]),
         "rvsp_hip_plan_numeric": (C.c_int, [V, V, V, V, V]),
+        "rvsp_hip_plan_fill_numeric": (C.c_int, [V, V, V, V, V]),
         "rvsp_hip_plan_check": (C.c_int, [V]),
         "rvsp_hip_plan_prepare_reuse": (C.c_int, [V, V]),
         "rvsp_hip_plan_get_bins": (C.c_int, [V, V, V]),
@@ -482,6 +483,14 @@ class Plan:
 
     def fill(self):
         check("plan_fill", lib().rvsp_hip_plan_fill(self.h, self.cci.ptr))
+        check("plan_check", lib().rvsp_hip_plan_check(self.h))
+        self.filled = True
+
+    def fill_numeric(self):
+        """Structure and values in one call (big rows' columns written by
+        the numeric pass)."""
+        check("plan_fill_numeric", lib().rvsp_hip_plan_fill_numeric(
+            self.h, self.dA.va.ptr, self.dB.va.ptr, self.cci.ptr, self.cval.ptr))
         check("plan_check", lib().rvsp_hip_plan_check(self.h))
         self.filled = True
 

@@ -307,21 +307,30 @@ rvsp_status_t rvsp_spgemm_compute(rvsp_spgemm_descr_t descr, const rvsp_csr_matr
             return s;
         if (refill)
         {
-            if ((s = rvsp_hip_plan_fill(plan, C->col_idx)))
+            /* structure and values in one go: big rows get their columns
+               from the numeric pass */
+            s = rvsp_hip_plan_fill_numeric(plan, (const float *)A->values,
+                                           (const float *)B->values, C->col_idx,
+                                           (float *)C->values);
+            if (s != RVSP_SUCCESS)
                 return s;
             descr->filled_into = C->col_idx;
         }
-        else if ((s = rvsp_hip_plan_prepare_reuse(plan, C->col_idx)))
-            return s;
-        if (descr->use_graph && !refill)
-            s = rvsp_hip_plan_numeric_graph(plan, (const float *)A->values,
-                                            (const float *)B->values, C->col_idx,
-                                            (float *)C->values);
         else
-            s = rvsp_hip_plan_numeric(plan, (const float *)A->values, (const float *)B->values,
-                                      C->col_idx, (float *)C->values);
-        if (s != RVSP_SUCCESS)
-            return s;
+        {
+            if ((s = rvsp_hip_plan_prepare_reuse(plan, C->col_idx)))
+                return s;
+            if (descr->use_graph)
+                s = rvsp_hip_plan_numeric_graph(plan, (const float *)A->values,
+                                                (const float *)B->values, C->col_idx,
+                                                (float *)C->values);
+            else
+                s = rvsp_hip_plan_numeric(plan, (const float *)A->values,
+                                          (const float *)B->values, C->col_idx,
+                                          (float *)C->values);
+            if (s != RVSP_SUCCESS)
+                return s;
+        }
     }
     else
     {
@@ -332,20 +341,25 @@ rvsp_status_t rvsp_spgemm_compute(rvsp_spgemm_descr_t descr, const rvsp_csr_matr
         memcpy(C->row_ptr, descr->h_row_ptr, rp_bytes);
         if (refill)
         {
-            if ((s = rvsp_hip_plan_fill(plan, descr->d_cci)) ||
+            /* columns go down after the pass that writes the big rows' */
+            if ((s = rvsp_hip_plan_fill_numeric(plan, descr->d_aval, descr->d_bval,
+                                                descr->d_cci, descr->d_cval)) ||
                 (s = rvsp_hip_copy_async(C->col_idx, descr->d_cci, c_bytes, st)))
                 return s;
         }
-        else if ((s = rvsp_hip_plan_prepare_reuse(plan, descr->d_cci)))
-            return s;
-        if (descr->use_graph && !refill)
-            s = rvsp_hip_plan_numeric_graph(plan, descr->d_aval, descr->d_bval, descr->d_cci,
-                                            descr->d_cval);
         else
-            s = rvsp_hip_plan_numeric(plan, descr->d_aval, descr->d_bval, descr->d_cci,
-                                      descr->d_cval);
-        if (s != RVSP_SUCCESS)
-            return s;
+        {
+            if ((s = rvsp_hip_plan_prepare_reuse(plan, descr->d_cci)))
+                return s;
+            if (descr->use_graph)
+                s = rvsp_hip_plan_numeric_graph(plan, descr->d_aval, descr->d_bval,
+                                                descr->d_cci, descr->d_cval);
+            else
+                s = rvsp_hip_plan_numeric(plan, descr->d_aval, descr->d_bval, descr->d_cci,
+                                          descr->d_cval);
+            if (s != RVSP_SUCCESS)
+                return s;
+        }
         if ((s = rvsp_hip_copy_async(C->values, descr->d_cval, c_bytes, st)))
             return s;
         if ((s = rvsp_hip_plan_check(plan)))

@@ -407,7 +407,10 @@ struct PieceGroup {
     }
 };
 
-template <int NW, int PW, int MAXCH>
+// COLS: the pass also writes each window's column indices (g.cols_out),
+// expanded from the rank dictionary into the accumulator's LDS once the
+// values are out, so a one-shot needs no separate big-row structure fill.
+template <int NW, int PW, int MAXCH, bool COLS>
 __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, BigArgs g)
 {
     constexpr int MAXW = MAXCH * kChunkWords;
@@ -556,6 +559,24 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
         wave_sync();
         for (int i = lane; i < outs; i += 64) a.cval[rowbase + o0 + i] = acc[i];
         wave_sync();
+        if (COLS) {
+            int32_t *stage = reinterpret_cast<int32_t *>(acc);
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int i = lane * PER + j;
+                uint32_t x = W[i];  // zero past the window's words
+                int pos = P[i];
+                while (x) {
+                    const int b = __ffs(x) - 1;
+                    x &= x - 1;
+                    if (pos < PW) stage[pos] = c0 + i * 32 + b;
+                    ++pos;
+                }
+            }
+            wave_sync();
+            for (int i = lane; i < outs && i < PW; i += 64) g.cols_out[rowbase + o0 + i] = stage[i];
+            wave_sync();
+        }
     }
 }
 
@@ -621,7 +642,10 @@ template <int PW, int MAXCH>
 static void launch_num_big_shape(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g)
 {
     constexpr int NW = 2;
-    k_num_big<NW, PW, MAXCH><<<(nsegs + NW - 1) / NW, NW * 64, 0, st>>>(nsegs, a, g);
+    if (g.cols_out)
+        k_num_big<NW, PW, MAXCH, true><<<(nsegs + NW - 1) / NW, NW * 64, 0, st>>>(nsegs, a, g);
+    else
+        k_num_big<NW, PW, MAXCH, false><<<(nsegs + NW - 1) / NW, NW * 64, 0, st>>>(nsegs, a, g);
 }
 
 void launch_num_big(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g)

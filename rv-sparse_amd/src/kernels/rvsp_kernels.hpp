@@ -43,6 +43,7 @@ struct BigArgs {
     int win_outs;         // numeric window: at most this many outputs ...
     int win_chunks;       // ... and this many chunks (one of the launchable shapes)
     int32_t nsegs;        // segments planned (set once the analysis has read the count)
+    int32_t *cols_out;    // numeric pass also writes the big rows' columns here (or null)
 };
 
 // Launchable big-row window shapes (outputs, chunks); the first is the default.
@@ -82,8 +83,9 @@ void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int
 // exceeds H/2 reports H + 1 instead of raising an error).
 void launch_sym_count(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
                       const int32_t *off, bool learned);
+// skip_big: the big rows' columns are left to a numeric pass with cols_out set.
 void launch_sym_fill(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
-                     const int32_t *off);
+                     const int32_t *off, bool skip_big = false);
 void launch_scan(hipStream_t st, int32_t m, const int32_t *cnt, int64_t *tsum, int64_t *total,
                  int32_t *crp);
 int scan_tiles(int32_t m);

@@ -658,18 +658,20 @@ static NumArgs num_args(rvsp_hip_plan *p, const float *aval, const float *bval, 
 }
 
 static void queue_numeric(rvsp_hip_plan *p, const float *aval, const float *bval,
-                          const int32_t *cci, float *cval)
+                          const int32_t *cci, float *cval, int32_t *big_cols = nullptr)
 {
+    BigArgs big = p->big;
+    big.cols_out = big_cols;
     // kernels that gather B's columns read the (column, value) pairs; the
     // rank kernels read only values, so a pass with ranks and no big rows
     // skips the pairs
     if (!p->pm_ready || p->nseg > 0) launch_interleave(p->stream, p->nnz_b, p->bci, bval, p->bcv);
     const NumArgs a = num_args(p, aval, bval, cci, cval);
     if (p->pm_ready)
-        launch_numeric_ranked(p->stream, a, p->nperm, p->num_count, p->num_off, p->big, p->nseg,
+        launch_numeric_ranked(p->stream, a, p->nperm, p->num_count, p->num_off, big, p->nseg,
                               p->prow, p->pmap);
     else
-        launch_numeric(p->stream, a, p->nperm, p->num_count, p->num_off, p->big, p->nseg);
+        launch_numeric(p->stream, a, p->nperm, p->num_count, p->num_off, big, p->nseg);
 }
 
 rvsp_status_t rvsp_hip_plan_numeric(rvsp_hip_plan_t *p, const float *aval, const float *bval,
@@ -679,6 +681,24 @@ rvsp_status_t rvsp_hip_plan_numeric(rvsp_hip_plan_t *p, const float *aval, const
     if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
     PhaseScope timer(p->stream, PH_NUMERIC);
     queue_numeric(p, aval, bval, cci, cval);
+    HIP_TRY(hipGetLastError());
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_plan_fill_numeric(rvsp_hip_plan_t *p, const float *aval,
+                                         const float *bval, int32_t *cci, float *cval)
+{
+    if (!p || !aval || !bval || !cci || !cval) return RVSP_ERROR_NULL_POINTER;
+    if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
+    {
+        SymArgs sa{p->arp, p->aci, p->brp, p->bci, p->lo,  p->hi,  p->cnt,
+                   p->crp, cci,    p->big, p->err, p->ops, p->bms_spill_ok ? p->bms_spill : nullptr};
+        PhaseScope timer(p->stream, PH_FILL);
+        launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off, /*skip_big=*/true);
+        HIP_TRY(hipGetLastError());
+    }
+    PhaseScope timer(p->stream, PH_NUMERIC);
+    queue_numeric(p, aval, bval, cci, cval, cci);
     HIP_TRY(hipGetLastError());
     return RVSP_SUCCESS;
 }

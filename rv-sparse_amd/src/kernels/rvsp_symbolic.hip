@@ -535,11 +535,12 @@ static void launch_sym_learned(hipStream_t st, const SymArgs &a, int b, const in
 
 template <bool FILL>
 static void launch_sym_bins(hipStream_t st, const SymArgs &a, const int32_t *perm,
-                            const int32_t *count, const int32_t *off, bool learned = false)
+                            const int32_t *count, const int32_t *off, bool learned = false,
+                            bool skip_big = false)
 {
     for (int b = SB_H64; b < SB_COUNT; ++b) {
         const int nr = count[b];
-        if (nr == 0) continue;
+        if (nr == 0 || (skip_big && b == SB_BIG)) continue;
         const int32_t *rows = perm + off[b];
         if (!FILL && learned && b >= SB_H64 && b <= SB_H4096) {
             launch_sym_learned(st, a, b, rows, nr);
@@ -593,9 +594,9 @@ void launch_sym_count(hipStream_t st, const SymArgs &a, const int32_t *perm, con
 }
 
 void launch_sym_fill(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
-                     const int32_t *off)
+                     const int32_t *off, bool skip_big)
 {
-    launch_sym_bins<true>(st, a, perm, count, off);
+    launch_sym_bins<true>(st, a, perm, count, off, false, skip_big);
 }
 
 void launch_scan(hipStream_t st, int32_t m, const int32_t *cnt, int64_t *tsum, int64_t *total,

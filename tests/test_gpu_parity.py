@@ -237,6 +237,33 @@ def test_big_rows_every_window_shape_and_count_block(shape, threads, monkeypatch
     check_product(A2, B2, rvsp.spgemm_raw(A2, B2), f"wide-700 {shape} threads={threads}")
 
 
+@pytest.mark.parametrize("shape", ["2048x16", "4096x32", "1024x8"])
+def test_fill_numeric_matches_fill_then_numeric(shape, monkeypatch):
+    """The combined call (big rows' columns written by the numeric pass)
+    leaves exactly what fill followed by numeric leaves, column indices and
+    value bits, including over a C buffer holding stale data."""
+    monkeypatch.setenv("RVSP_BIG_WINDOW", shape)
+    for name, (A, B) in [("rmat-15", W.rmat(scale=15, seed=31)),
+                         ("wide-700", (rvsp.gen_random_rows(40, 3000, 700, 700, 78),
+                                       rvsp.gen_random_rows(3000, 100000, 250, 350, 77)))]:
+        say(f"fill+numeric {name} {shape}")
+        dA, dB = rvsp.DeviceCsr(A), rvsp.DeviceCsr(B)
+        sep = rvsp.Plan(dA, dB)
+        sep.fill()
+        sep.numeric()
+        ref = sep.result()
+        one = rvsp.Plan(dA, dB)
+        one.cci.upload(np.full(max(one.nnz, 1), -7, np.int32))
+        one.cval.upload(np.full(max(one.nnz, 1), np.nan, np.float32))
+        one.fill_numeric()
+        got = one.result()
+        np.testing.assert_array_equal(got.col_idx, ref.col_idx)
+        np.testing.assert_array_equal(got.values.view(np.uint32), ref.values.view(np.uint32))
+        check_product(A, B, got, f"{name} fill+numeric {shape}")
+        sep.close()
+        one.close()
+
+
 # ---------------------------------------------------------------------------
 # edge cases
 # ---------------------------------------------------------------------------
