@@ -13,11 +13,13 @@
 //     execute in issue order, so successive k-steps reach an accumulator
 //     slot in k order. A compiler fence between k-steps keeps the issue
 //     order equal to program order.
-// Rows are binned (exact nnz(C_i) from the symbolic pass):
-//   hash    nnz(C_i) <= 2048: LDS table of H >= 2 nnz slots, pre-loaded with
-//           C_i's columns, G lanes per row (G ~ mean |B_k|)
-//   dense   an LDS window of W >= span columns indexed by column, when
-//           it is at most twice the hash table's bytes
+// Rows are binned (exact nnz(C_i) from the symbolic pass) to the accumulator
+// with the smallest LDS footprint (or the learned chooser's pick):
+//   hash    nnz(C_i) <= 2730: LDS table of H >= 1.5 nnz slots, pre-loaded
+//           with C_i's columns, G lanes per row (G ~ mean |B_k|)
+//   dense   an LDS window of W >= span columns indexed by column
+//   rank    span <= 16384 and nnz(C_i) <= 4096: a bitmap of C_i's columns
+//           with a popcount prefix maps a column to its output position
 //   big     rows the symbolic pass kept a bitmap for (rvsp_bigrow.hip)
 #include "rvsp_device.hpp"
 #include "rvsp_kernels.hpp"
