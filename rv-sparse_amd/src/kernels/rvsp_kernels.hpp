@@ -90,10 +90,11 @@ void launch_scan(hipStream_t st, int32_t m, const int32_t *cnt, int64_t *tsum, i
                  int32_t *crp);
 int scan_tiles(int32_t m);
 // chooser: device parameters of the learned accumulator chooser, or null
-// for the heuristic.
+// for the heuristic; lg_margin: log2 of the predicted speed-up the chooser
+// needs to leave the heuristic's accumulator.
 void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_t *ops,
                      const int32_t *arp, const uint8_t *sbin, const int32_t *lo,
-                     const int32_t *hi, const float *chooser, uint8_t *nbin,
+                     const int32_t *hi, const float *chooser, float lg_margin, uint8_t *nbin,
                      AnalysisCounters *ctr);
 void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
                     const int32_t *off, const BigArgs &big, int nsegs);

@@ -91,12 +91,16 @@ __device__ __forceinline__ int chooser_bin(int o, int nz, int64_t span)
 }
 
 // The feasible accumulator with the smallest predicted time (-1: none).
-__device__ inline int chooser_pick(const float *P, int64_t ops, int alen, int64_t span, int nz)
+// When the heuristic's pick `hb` is one of the outputs, the chooser keeps it
+// unless the best prediction is faster by more than lg_margin (log2 of the
+// chooser's margin: 0 keeps the plain argmin).
+__device__ inline int chooser_pick(const float *P, int64_t ops, int alen, int64_t span, int nz,
+                                   int hb, float lg_margin)
 {
     float f[kFeat], z1[16], h1[16], z2[16], h2[16], y[8];
     row_features(ops, alen, span, nz, f);
     Chooser::forward(P, f, z1, h1, z2, h2, y);
-    int best = -1;
+    int best = -1, oh = -1;
     float by = 0.0f;
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
@@ -105,7 +109,9 @@ __device__ inline int chooser_pick(const float *P, int64_t ops, int alen, int64_
             best = b;
             by = y[o];
         }
+        if (b >= 0 && b == hb) oh = o;
     }
+    if (oh >= 0 && !(by + lg_margin < y[oh])) return hb;
     return best;
 }
 

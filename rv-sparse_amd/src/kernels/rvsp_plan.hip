@@ -445,11 +445,11 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     HIP_TRY(hipMemsetAsync(p->err, 0, 4, st));
 
     // learned selectors (null when none is loaded: heuristic rules)
-    float margin = 1.0f, unused = 1.0f;
+    float margin = 1.0f, cmargin = 1.0f;
     const float *sizer =
         select == RVSP_SELECT_LEARNED ? selector_device_params(RVSP_SELECTOR_SIZER, &margin) : nullptr;
     const float *chooser =
-        select == RVSP_SELECT_LEARNED ? selector_device_params(RVSP_SELECTOR_CHOOSER, &unused) : nullptr;
+        select == RVSP_SELECT_LEARNED ? selector_device_params(RVSP_SELECTOR_CHOOSER, &cmargin) : nullptr;
     p->sized_rows = p->fallback_rows = 0;
 
     // ---- 1. per-row product counts, spans and symbolic bins
@@ -593,7 +593,8 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
                         p->ctr);
         HIP_TRY(hipGetLastError());
     }
-    launch_num_bins(st, m, p->crp, p->ops, arp, p->sbin, p->lo, p->hi, chooser, p->nbin, p->ctr);
+    launch_num_bins(st, m, p->crp, p->ops, arp, p->sbin, p->lo, p->hi, chooser,
+                    cmargin > 1.0f ? log2f(cmargin) : 0.0f, p->nbin, p->ctr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&p->hctr, p->ctr, sizeof(AnalysisCounters), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));

@@ -468,6 +468,7 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
                                                   const int32_t *__restrict__ lo,
                                                   const int32_t *__restrict__ hi,
                                                   const float *__restrict__ chooser,
+                                                  float lg_margin,
                                                   uint8_t *__restrict__ nbin,
                                                   AnalysisCounters *__restrict__ ctr)
 {
@@ -488,8 +489,9 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
         else {
             const int alen = arp[i + 1] - arp[i];
             const int64_t span = (int64_t)hi[i] - lo[i] + 1;
-            b = chooser ? chooser_pick(P, ops[i], alen, span, nz) : -1;
-            if (b < 0) b = num_bin_of(nz, ops[i], alen, span);
+            const int hb = num_bin_of(nz, ops[i], alen, span);
+            b = chooser ? chooser_pick(P, ops[i], alen, span, nz, hb, lg_margin) : -1;
+            if (b < 0) b = hb;
         }
         nbin[i] = (uint8_t)b;
         atomicAdd(&hist[b], 1);
@@ -617,11 +619,12 @@ int scan_tiles(int32_t m)
 
 void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_t *ops,
                      const int32_t *arp, const uint8_t *sbin, const int32_t *lo,
-                     const int32_t *hi, const float *chooser, uint8_t *nbin,
+                     const int32_t *hi, const float *chooser, float lg_margin, uint8_t *nbin,
                      AnalysisCounters *ctr)
 {
     const int grid = (int)(((int64_t)m + 255) / 256);
-    k_num_bins<<<grid, 256, 0, st>>>(m, crp, ops, arp, sbin, lo, hi, chooser, nbin, ctr);
+    k_num_bins<<<grid, 256, 0, st>>>(m, crp, ops, arp, sbin, lo, hi, chooser, lg_margin, nbin,
+                                     ctr);
 }
 
 } // namespace rvsp
