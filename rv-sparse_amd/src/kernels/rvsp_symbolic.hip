@@ -345,10 +345,10 @@ __global__ __launch_bounds__(256) void k_sym_bitmap(const int32_t *__restrict__ 
 {
     constexpr int R = 256 / G;
     constexpr int COLS = 32 * WORDS;
-    constexpr int WPT = WORDS / G;  // words per thread in the ordered scan
+    constexpr int WPT = WORDS / G;  // words per lane (rounds of 64 in the fill)
     static_assert(WORDS % G == 0, "words per thread");
+    static_assert(G == 64, "one row per wave");
     __shared__ uint32_t bits[R * WORDS];
-    __shared__ int scan[256];
     const int g = threadIdx.x / G, lane = threadIdx.x % G;
     const int64_t ridx = (int64_t)blockIdx.x * R + g;
     const bool valid = ridx < nrows;
@@ -375,35 +375,38 @@ __global__ __launch_bounds__(256) void k_sym_bitmap(const int32_t *__restrict__ 
     __syncthreads();
     if (!FILL && spilled)
         for (int w = lane; w < WORDS; w += G) slot[w] = Bm[w];
-    // ordered scan: thread `lane` owns words [lane*WPT, (lane+1)*WPT)
-    int local = 0;
-#pragma unroll 4
-    for (int w = 0; w < WPT; ++w) local += __popc(Bm[lane * WPT + w]);
-    scan[threadIdx.x] = local;
-    __syncthreads();
-    for (int off = 1; off < G; off <<= 1) {
-        const int v = lane >= off ? scan[threadIdx.x - off] : 0;
-        __syncthreads();
-        scan[threadIdx.x] += v;
-        __syncthreads();
-    }
-    const int incl = scan[threadIdx.x];
-    const int gtotal = scan[g * G + G - 1];
+    if (!valid) return;  // no block barriers below: one row per wave
     if constexpr (FILL) {
-        if (valid) {
-            int pos = crp[row] + incl - local;
-            for (int w = 0; w < WPT; ++w) {
-                uint32_t word = Bm[lane * WPT + w];
-                const int32_t cb = base + (lane * WPT + w) * 32;
-                while (word) {
-                    const int b = __ffs(word) - 1;
-                    word &= word - 1;
-                    cci[pos++] = cb + b;
-                }
+        // ascending columns, 64 words per round (lane l: word 64r + l): a
+        // wave scan of the round's popcounts places each lane's bits, so one
+        // store instruction covers a short run of C_i and no lane expands
+        // more than one word per round
+        int run = crp[row];
+#pragma unroll 1
+        for (int r = 0; r < WPT; ++r) {
+            uint32_t x = Bm[r * 64 + lane];
+            const int c = __popc(x);
+            int incl = c;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int y = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += y;
             }
+            int pos = run + incl - c;
+            const int32_t cb = base + (r * 64 + lane) * 32;
+            while (x) {
+                const int b = __ffs(x) - 1;
+                x &= x - 1;
+                cci[pos++] = cb + b;
+            }
+            run += lane_bcast(incl, 63);
         }
     } else {
-        if (valid && lane == 0) cnt[row] = gtotal;
+        int local = 0;
+#pragma unroll
+        for (int w = 0; w < WPT; ++w) local += __popc(Bm[w * 64 + lane]);
+        const int gtotal = group_sum<64>(local);
+        if (lane == 0) cnt[row] = gtotal;
     }
 }
 
