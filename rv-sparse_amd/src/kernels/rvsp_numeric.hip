@@ -94,6 +94,29 @@ __device__ __forceinline__ void walk_row_ordered(const NumArgs &a, int as, int a
     }
 }
 
+// Calls fn(q, cci[q]) for every q in [cs, ce) with the G lanes of a row,
+// eight loads per lane issued before any call, so reading C_i's columns
+// costs one dependent load per 8 G columns instead of one per G.
+template <int G, typename Fn>
+__device__ __forceinline__ void for_each_c_col(const int32_t *__restrict__ cci, int cs, int ce,
+                                               Fn fn)
+{
+    const int lane = threadIdx.x % G;
+    for (int q0 = cs; q0 < ce; q0 += 8 * G) {
+        int32_t cc[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = q0 + u * G + lane;
+            cc[u] = q < ce ? cci[q] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = q0 + u * G + lane;
+            if (q < ce) fn(q, cc[u]);
+        }
+    }
+}
+
 template <int G, int H, int R>
 __global__ __launch_bounds__(R *G) void k_num_hash(const int32_t *__restrict__ rows, int32_t nrows,
                                                    NumArgs a)
@@ -113,7 +136,7 @@ __global__ __launch_bounds__(R *G) void k_num_hash(const int32_t *__restrict__ r
     }
     __syncthreads();
     const int cs = valid ? a.crp[row] : 0, ce = valid ? a.crp[row + 1] : 0;
-    for (int q = cs + lane; q < ce; q += G) hash_insert_slot<H>(T, a.cci[q], a.err);
+    for_each_c_col<G>(a.cci, cs, ce, [&](int, int32_t c) { hash_insert_slot<H>(T, c, a.err); });
     __syncthreads();
     if (valid) {
         int *err = a.err;
@@ -124,10 +147,10 @@ __global__ __launch_bounds__(R *G) void k_num_hash(const int32_t *__restrict__ r
                             });
     }
     __syncthreads();
-    for (int q = cs + lane; q < ce; q += G) {
-        const int s = hash_find<H>(T, a.cci[q], a.err);
+    for_each_c_col<G>(a.cci, cs, ce, [&](int q, int32_t c) {
+        const int s = hash_find<H>(T, c, a.err);
         a.cval[q] = s >= 0 ? V[s] : 0.0f;
-    }
+    });
 }
 
 // Dense window: G lanes per row, R rows per block, an LDS accumulator of W
@@ -160,10 +183,10 @@ __global__ __launch_bounds__(R *G) void k_num_dense(const int32_t *__restrict__ 
     }
     __syncthreads();
     const int cs = valid ? a.crp[row] : 0, ce = valid ? a.crp[row + 1] : 0;
-    for (int q = cs + lane; q < ce; q += G) {
-        const int32_t c = a.cci[q] - wlo;
+    for_each_c_col<G>(a.cci, cs, ce, [&](int q, int32_t col) {
+        const int32_t c = col - wlo;
         a.cval[q] = (uint32_t)c < (uint32_t)W ? S[c] : 0.0f;
-    }
+    });
 }
 
 // Rank window: G lanes per row (32 or 64), R rows per block. The row's
@@ -194,13 +217,13 @@ __global__ __launch_bounds__(R *G) void k_num_rankw(const int32_t *__restrict__ 
     for (int i = lane; i < kRankWords; i += G) Wr[i] = 0u;
     for (int i = lane; i < nz; i += G) S[i] = 0.0f;
     __syncthreads();
-    for (int q = lane; q < nz; q += G) {
-        const int32_t c = a.cci[cs + q] - wlo;
+    for_each_c_col<G>(a.cci, cs, cs + nz, [&](int, int32_t col) {
+        const int32_t c = col - wlo;
         if ((uint32_t)c < (uint32_t)kRankMaxCols)
             atomicOr(&Wr[c >> 5], 1u << (c & 31));
         else
             atomicOr(a.err, kErrWindow);
-    }
+    });
     __syncthreads();
     {
         int local = 0;
