@@ -52,14 +52,27 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
     int64_t ops = 0;
     int32_t lo = INT32_MAX, hi = -1;
     if (row < m) {
+        // eight entries per lane per step: each of the three dependent loads
+        // (A column, B row bounds, B end columns) is issued for all eight
+        // before any is used, so a long row costs 3 round trips per 8 G entries
+        constexpr int U = 8;
         const int as = arp[row], ae = arp[row + 1];
-        for (int ap = as + lane; ap < ae; ap += G) {
-            const int k = aci[ap];
-            const int bs = brp[k], be = brp[k + 1];
-            ops += be - bs;
-            if (be > bs) {
-                lo = min(lo, bci[bs]);
-                hi = max(hi, bci[be - 1]);
+        for (int ap0 = as + lane; ap0 < ae; ap0 += U * G) {
+            int k[U], bs[U], be[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) k[u] = ap0 + u * G < ae ? aci[ap0 + u * G] : -1;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                bs[u] = k[u] >= 0 ? brp[k[u]] : 0;
+                be[u] = k[u] >= 0 ? brp[k[u] + 1] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ops += be[u] - bs[u];
+                if (be[u] > bs[u]) {
+                    lo = min(lo, bci[bs[u]]);
+                    hi = max(hi, bci[be[u] - 1]);
+                }
             }
         }
     }
