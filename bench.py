@@ -53,6 +53,14 @@ def load_workload(name):
     if name == "stencil":
         A, B = W.stencil()
         return A, B, True, "27-point stencil 64^3, C=A*A, N(0,1) FP32"
+    if name.startswith("mtx:"):
+        # a local Matrix Market file, C = A*A as rv-sparse's bench runs its
+        # real matrices (read by the library's rvsp_mtx_load)
+        path = name[4:]
+        A = rvsp.mtx_load(path)
+        if A.rows != A.cols:
+            raise SystemExit(f"{path}: {A.rows} x {A.cols} (inferred) is not square, no A*A")
+        return A, A, True, f"{os.path.basename(path)} ({A.rows} x {A.cols}, nnz {A.nnz}), C=A*A, file values FP32"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -196,7 +204,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rmat")
+    ap.add_argument("--workload", default="rmat",
+                    help="rmat, mixed, uniform, stencil, batched, or mtx:PATH (a local .mtx file, C=A*A)")
     ap.add_argument("--algo", default="learned", choices=["hip", "learned"],
                     help="learned: MLP row selectors (the plain heuristic is timed beside it)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,

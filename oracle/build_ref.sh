@@ -28,5 +28,11 @@ while IFS= read -r src; do
 done < <(find "$REF/src" -name '*.c' | sort)
 gcc -shared -Wl,-Bsymbolic -o "$OUT/librvsp_ref.so.tmp" "${objs[@]}" -lm
 mv "$OUT/librvsp_ref.so.tmp" "$OUT/librvsp_ref.so"
+# rv-sparse's Matrix Market reader (tools/src: the formatter and its vector),
+# the checker of rvsp_mtx_load
+gcc -std=c11 -O2 -fPIC -shared -w -I"$REF/tools/include" \
+    "$REF/tools/src/mtx_to_csr_formatter.c" "$REF/tools/src/vec.c" \
+    -o "$OUT/librvsp_mtx_ref.so.tmp"
+mv "$OUT/librvsp_mtx_ref.so.tmp" "$OUT/librvsp_mtx_ref.so"
 rm -rf "$OUT/obj"
-echo "build_ref: wrote $OUT/librvsp_ref.so"
+echo "build_ref: wrote $OUT/librvsp_ref.so and $OUT/librvsp_mtx_ref.so"

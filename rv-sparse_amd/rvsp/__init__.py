@@ -76,6 +76,7 @@ def _declare(lib):
         "rvsp_csr_create": (C.c_int, [CSRP, I32, I32, I32, V, V, V, C.c_int]),
         "rvsp_csr_validate": (C.c_int, [CSRP]),
         "rvsp_csr_destroy": (None, [CSRP]),
+        "rvsp_mtx_load": (C.c_int, [C.c_char_p, CSRP]),
         "rvsp_spgemm_csr": (C.c_int, [CSRP, CSRP, CSRP, P(Options)]),
         "rvsp_spgemm_descr_create": (C.c_int, [P(V)]),
         "rvsp_spgemm_descr_destroy": (None, [V]),
@@ -388,6 +389,30 @@ class DeviceCsr:
     def struct(self) -> CsrStruct:
         return CsrStruct(self.rows, self.cols, self.nnz, self.rp.ptr, self.ci.ptr, self.va.ptr,
                          DTYPE_FP32, FORMAT_CSR, 0)
+
+
+def mtx_load(path: str, raise_on_error=True):
+    """rvsp_mtx_load: a Matrix Market file as a host Csr (FP32, canonical),
+    or (status, None) with raise_on_error=False on failure."""
+    c = CsrStruct()
+    st = lib().rvsp_mtx_load(str(path).encode(), C.byref(c))
+    if st != SUCCESS:
+        if raise_on_error:
+            check("mtx_load", st)
+        return st, None
+    n = c.nnz
+
+    def host(ptr, ct, count, dt):
+        if not count:
+            return np.zeros(0, dt)
+        return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(count,)).copy()
+
+    rp = host(c.row_ptr, C.c_int32, c.rows + 1, np.int32)
+    ci = host(c.col_idx, C.c_int32, n, np.int32)
+    va = host(c.values, C.c_float, n, np.float32)
+    out = Csr(c.rows, c.cols, rp, ci, va)
+    lib().rvsp_csr_destroy(C.byref(c))
+    return out if raise_on_error else (SUCCESS, out)
 
 
 def device_csr_to_host(c: CsrStruct) -> Csr:
