@@ -199,6 +199,106 @@ def copy_probe(mib):
     dst.free()
 
 
+# rv-sparse's per-run CSV schema (bench/bench.c:531-534) and field formats
+# (bench.c:879-913), so GPU rows land in one table with the reference's rows
+CSV_HEADER = ("label,kernel,arm,build,march,cflags,cc_version,dtype,"
+              "rows,cols,nnz_a,nnz_b,nnz_c,flops,op_mean,op_max,op_var,"
+              "run,time_s,gops,correct,cycles,instructions")
+
+
+def csv_row(meta, run, dt, nnz_c, correct):
+    gops = meta["flops"] / dt / 1e9 if dt > 0 else 0.0
+    return (f"{meta['label']},{meta['kernel']},{meta['arm']},{meta['build']},{meta['march']},"
+            f"{meta['cflags']},{meta['cc_version']},f32,"
+            f"{meta['rows']},{meta['cols']},{meta['nnz_a']},{meta['nnz_b']},{nnz_c},"
+            f"{meta['flops']:.0f},{meta['op_mean']:.4f},{meta['op_max']:.0f},{meta['op_var']:.4f},"
+            f"{run},{dt:.9f},{gops:.4f},{correct},,")  # no perf counters: empty, as bench.c
+
+
+def bench_csv(args):
+    """Rows in rv-sparse's bench CSV schema, one per timed run, for the GPU
+    raw one-shot on host arrays (hip_f32, PCIe included: what rv-sparse's
+    bench times once the kernel is bound, bench.c:853-866), the device
+    one-shot with inputs resident in HBM (hip_f32_dev), and rv-sparse's own
+    scalar kernel on the host cores (scalar_f32, the CPU leg). `correct` is
+    1 when C matches the scalar kernel's result (bench.c:316-352)."""
+    import subprocess
+    from oracle import pyoracle
+
+    A, B, same, desc = load_workload(args.workload)
+    L = rvsp.lib()
+    d = rvsp.Descr()
+    st, _, _ = d.estimate(A.struct(), B.struct())
+    rvsp.check("estimate", st)
+    st, ops = d.op_counts(A.rows)
+    rvsp.check("op_counts", st)
+    d.close()
+    ops = ops.astype(np.float64)
+    mean = float(ops.mean())
+    meta = {"label": args.csv_label, "rows": A.rows, "cols": A.cols, "nnz_a": A.nnz, "nnz_b": B.nnz,
+            "flops": 2.0 * float(ops.sum()), "op_mean": mean, "op_max": float(ops.max()),
+            "op_var": float(((ops - mean) ** 2).mean())}
+    try:
+        ccv = subprocess.run(["hipcc", "--version"], capture_output=True, text=True,
+                             timeout=60).stdout.split("\n")[0].replace(",", ";").strip() or "-"
+    except Exception:
+        ccv = "-"
+    kind = "reference" if pyoracle.ref_available() else "port"
+    ref = (pyoracle.ref_spgemm(A, B) if kind == "reference" else pyoracle.spgemm(A, B))[:3]
+
+    def same_as_ref(rp, ci, va):
+        return int(np.array_equal(rp, ref[0]) and np.array_equal(ci, ref[1]) and
+                   np.array_equal(va.view(np.uint32), np.asarray(ref[2], np.float32).view(np.uint32)))
+
+    out = [CSV_HEADER]
+    times = {}
+    gpu_meta = dict(meta, arm="hip", build="gfx950", march="gfx950",
+                    cflags="-O3 -ffp-contract=off", cc_version=ccv)
+    # GPU, host arrays in and out
+    for _ in range(args.warmup):
+        rvsp.spgemm_raw(A, B)
+    for r in range(args.steps):
+        Cg, dt = rvsp.spgemm_raw(A, B, timed=True)  # the library call alone, as bench.c times
+        out.append(csv_row(dict(gpu_meta, kernel="hip_f32"), r, dt, Cg.nnz,
+                           same_as_ref(Cg.row_ptr, Cg.col_idx, Cg.values)))
+        times.setdefault("hip_f32", []).append(dt)
+    # GPU, device-resident operands
+    dA = rvsp.DeviceCsr(A)
+    dB = dA if same else rvsp.DeviceCsr(B)
+    a, b = dA.struct(), dB.struct()
+    for r in range(-args.warmup, args.steps):
+        c = rvsp.CsrStruct()
+        rvsp.check("sync", L.rvsp_hip_synchronize())
+        t0 = time.perf_counter()
+        rvsp.check("csr_device", L.rvsp_spgemm_csr_device_algo(C.byref(a), C.byref(b), C.byref(c),
+                                                               rvsp.ALGO_HIP_LEARNED))
+        rvsp.check("sync", L.rvsp_hip_synchronize())
+        dt = time.perf_counter() - t0
+        if r >= 0:
+            h = rvsp.device_csr_to_host(c)
+            out.append(csv_row(dict(gpu_meta, kernel="hip_f32_dev"), r, dt, c.nnz,
+                               same_as_ref(h.row_ptr, h.col_idx, h.values)))
+            times.setdefault("hip_f32_dev", []).append(dt)
+        L.rvsp_csr_destroy(C.byref(c))
+    # CPU: rv-sparse's scalar kernel (or the oracle's restatement), 1 thread
+    cpu_meta = dict(meta, kernel="scalar_f32", arm="baseline", build="x86-64-v3",
+                    march="x86-64-v3", cflags="-std=c11 -O3", cc_version="gcc")
+    for r in range(args.csv_cpu_runs):
+        if kind == "reference":
+            rp, ci, va, dt = pyoracle.ref_spgemm(A, B, timed=True)
+            out.append(csv_row(cpu_meta, r, dt, int(rp[-1]), same_as_ref(rp, ci, va)))
+        else:
+            dt = pyoracle.time_oneshot(A, B)
+            out.append(csv_row(dict(cpu_meta, kernel="orc_scalar_f32"), r, dt, int(ref[0][-1]), 1))
+        times.setdefault("scalar_f32", []).append(dt)
+    with open(args.csv, "w") as f:
+        f.write("\n".join(out) + "\n")
+    print(json.dumps({"csv": args.csv, "workload": desc, "rows": len(out) - 1,
+                      "median_s": {k: round(float(np.median(v)), 6) for k, v in times.items()},
+                      "median_gflops": {k: round(meta["flops"] / float(np.median(v)) / 1e9, 3)
+                                        for k, v in times.items()}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +315,11 @@ def main():
                     help="also time heuristic vs learned vs exhaustive-best numeric bins")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--csv", default=None,
+                    help="instead of the JSON bench line: write rv-sparse bench CSV rows (one per "
+                         "timed run) for hip_f32, hip_f32_dev and scalar_f32 to this file")
+    ap.add_argument("--csv-label", default="-")
+    ap.add_argument("--csv-cpu-runs", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -225,6 +330,8 @@ def main():
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
 
+    if args.csv:
+        return bench_csv(args)
     if args.workload == "stencil":
         return bench_reuse(args)
     if args.workload == "batched":

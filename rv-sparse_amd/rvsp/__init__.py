@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -255,8 +256,9 @@ def spgemm_csr(A: Csr, B: Csr, backend: int = BACKEND_HIP, raise_on_error=True):
     return st, _from_owned_host(c)
 
 
-def spgemm_raw(A: Csr, B: Csr, want_ops=False):
-    """rvsp_spgemm_hip_f32, the raw one-shot kernel with callee-malloc'd C."""
+def spgemm_raw(A: Csr, B: Csr, want_ops=False, timed=False):
+    """rvsp_spgemm_hip_f32, the raw one-shot kernel with callee-malloc'd C;
+    timed=True also returns the seconds of the library call alone."""
     L = lib()
     ws_bytes = C.c_size_t(0)
     check("buffer_size", L.rvsp_spgemm_buffer_size(B.cols, C.byref(ws_bytes)))
@@ -264,6 +266,7 @@ def spgemm_raw(A: Csr, B: Csr, want_ops=False):
     rp, ci, va = C.c_void_p(), C.c_void_p(), C.c_void_p()
     nnz = C.c_int32(0)
     ops = np.zeros(A.rows, np.int64) if want_ops else None
+    t0 = time.perf_counter()
     st = L.rvsp_spgemm_hip_f32(
         A.rows, A.cols, B.cols,
         A.row_ptr.ctypes.data, A.col_idx.ctypes.data, A.values.ctypes.data,
@@ -272,10 +275,12 @@ def spgemm_raw(A: Csr, B: Csr, want_ops=False):
         C.byref(rp), C.byref(ci), C.byref(va), C.byref(nnz),
         ops.ctypes.data if want_ops else None,
     )
+    dt = time.perf_counter() - t0
     check("rvsp_spgemm_hip_f32", st)
     c = CsrStruct(A.rows, B.cols, nnz.value, rp.value, ci.value, va.value, DTYPE_FP32, FORMAT_CSR, 1)
     out = _from_owned_host(c)
-    return (out, ops) if want_ops else out
+    res = (out,) + ((ops,) if want_ops else ()) + ((dt,) if timed else ())
+    return res if len(res) > 1 else out
 
 
 class Descr:
