@@ -76,6 +76,25 @@ rvsp_status_t rvsp_hip_plan_fill_numeric(rvsp_hip_plan_t *plan,
                                          int32_t *c_col_idx,
                                          float *c_values);
 
+/* Numeric pass in one of rv-sparse's legacy dtypes, after the structure
+ * fill into c_col_idx: in_dtype RVSP_DTYPE_FP64 (double values in and out)
+ * or RVSP_DTYPE_INT8 (int8 values in, int32 accumulation and values out).
+ * Same ascending-k accumulation as the FP32 pass (FP64 products and sums
+ * each rounded once; INT32 wraps). Queued on the plan stream. */
+rvsp_status_t rvsp_hip_plan_numeric_typed(rvsp_hip_plan_t *plan, rvsp_dtype_t in_dtype,
+                                          const void *a_values, const void *b_values,
+                                          const int32_t *c_col_idx, void *c_values);
+
+/* Drops the entries of C whose value is exactly zero (value_dtype FP64 or
+ * INT32), as rv-sparse's legacy kernels store only nonzero results: writes
+ * the new row pointer (m + 1), columns and values into the device arrays
+ * rp_out / ci_out / cv_out (sized like C) and the kept count into *nnz_out
+ * (host; the call synchronises the plan stream). */
+rvsp_status_t rvsp_hip_plan_drop_zeros(rvsp_hip_plan_t *plan, rvsp_dtype_t value_dtype,
+                                       const int32_t *c_col_idx, const void *c_values,
+                                       int32_t *rp_out, int32_t *ci_out, void *cv_out,
+                                       int32_t *nnz_out);
+
 /* Same numeric pass, captured once into a hipGraph and replayed while the
  * four pointers stay the same (re-captured when they change). */
 rvsp_status_t rvsp_hip_plan_numeric_graph(rvsp_hip_plan_t *plan,

@@ -7,8 +7,9 @@
  * sequence of spgemm.c:26-133 (dtype and shape checks, canonical check,
  * descriptor estimate, library-allocated C, compute, cleanup on failure).
  * FP32 SCALAR/SCALAR_UNROLL4/HIP run on the GPU. RVV is unsupported here,
- * as in any non-vector rv-sparse build. The FP64 and INT8 legacy kernels
- * are not part of this build and report RVSP_ERROR_UNSUPPORTED_BACKEND.
+ * as in any non-vector rv-sparse build. The legacy dtypes (FP64 in and out,
+ * INT8 in with INT32 out; rv-sparse spgemm.c:182-194) run on the GPU too,
+ * through spgemm_legacy.c; their RVV variants stay unsupported.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -17,6 +18,10 @@
 #include "rv_sparse_hip.h"
 #include "rvsp_hip_abi.h"
 #include "rvsp_spgemm.h"
+
+/* spgemm_legacy.c */
+rvsp_status_t rvsp_spgemm_legacy_hip(const rvsp_csr_matrix_t *A, const rvsp_csr_matrix_t *B,
+                                     rvsp_csr_matrix_t *C, rvsp_dtype_t in);
 
 static void reset_c(rvsp_csr_matrix_t *C)
 {
@@ -115,6 +120,13 @@ rvsp_status_t rvsp_spgemm_csr(const rvsp_csr_matrix_t *A, const rvsp_csr_matrix_
     if (backend == RVSP_BACKEND_RVV_INTRINSICS && A->dtype == RVSP_DTYPE_FP32 &&
         B->dtype == RVSP_DTYPE_FP32)
         return oneshot_host(A, B, C, RVSP_SPGEMM_ALGO_RVV); /* unsupported here */
+    /* legacy dtypes (spgemm.c:182-194): the scalar backend, or HIP by name */
+    if ((backend == RVSP_BACKEND_SCALAR || backend == RVSP_BACKEND_HIP) &&
+        in == RVSP_DTYPE_INT8 && out == RVSP_DTYPE_INT32)
+        return rvsp_spgemm_legacy_hip(A, B, C, RVSP_DTYPE_INT8);
+    if ((backend == RVSP_BACKEND_SCALAR || backend == RVSP_BACKEND_HIP) &&
+        in == RVSP_DTYPE_FP64 && out == RVSP_DTYPE_FP64)
+        return rvsp_spgemm_legacy_hip(A, B, C, RVSP_DTYPE_FP64);
     return RVSP_ERROR_UNSUPPORTED_BACKEND;
 }
 

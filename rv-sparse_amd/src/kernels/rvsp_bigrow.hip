@@ -22,6 +22,8 @@
 //            (gustavson_core_f32.inc:31-63, accum_scalar_f32.c:67-70).
 #include <stdlib.h>
 
+#include <type_traits>
+
 #include "rvsp_device.hpp"
 #include "rvsp_kernels.hpp"
 
@@ -309,9 +311,10 @@ __device__ __forceinline__ int lane_lower_bound(const int32_t *__restrict__ a, i
 
 // Adds one batch of a piece (lanes hold consecutive entries of B_k, so
 // distinct columns) and returns how many of its entries lie in the window.
-__device__ __forceinline__ int big_batch(float aj, int32_t col, float bv, int32_t c0, int32_t c1,
+template <class Acc>
+__device__ __forceinline__ int big_batch(Acc aj, int32_t col, Acc bv, int32_t c0, int32_t c1,
                                          int outs, const uint32_t *W, const uint16_t *P,
-                                         float *acc, const NumArgs &a)
+                                         Acc *acc, int *err)
 {
     const bool in = col < c1;
     if (in) {
@@ -321,11 +324,11 @@ __device__ __forceinline__ int big_batch(float aj, int32_t col, float bv, int32_
             const uint32_t word = W[wi];
             const int pos = (int)P[wi] + __popc(word & ((1u << (rel & 31)) - 1u));
             if (((word >> (rel & 31)) & 1u) && pos < outs)
-                atomicAdd(&acc[pos], __fmul_rn(aj, bv));
+                atomicAdd(&acc[pos], vmul(aj, bv));
             else
-                atomicOr(a.err, kErrProbeMiss);
+                atomicOr(err, kErrProbeMiss);
         } else {
-            atomicOr(a.err, kErrWindow);
+            atomicOr(err, kErrWindow);
         }
     }
     return __popcll(__ballot(in));
@@ -338,29 +341,30 @@ __device__ __forceinline__ int big_batch(float aj, int32_t col, float bv, int32_
 // kPieceBatches batches loaded at once before their adds.
 constexpr int kPieceBatches = 4;
 
-__device__ __forceinline__ int2 big_piece(int cj, int ej, float aj, int32_t col, float bv,
-                                          int32_t c0, int32_t c1, int outs, const uint32_t *W,
-                                          const uint16_t *P, float *acc, const NumArgs &a,
-                                          const int2 *__restrict__ pairs)
+template <class Acc, class Pair>
+__device__ __forceinline__ int2 big_piece(int cj, int ej, Acc aj, int32_t col, Acc bv, int32_t c0,
+                                          int32_t c1, int outs, const uint32_t *W,
+                                          const uint16_t *P, Acc *acc, int *err,
+                                          const Pair *__restrict__ pairs)
 {
     const int lane = threadIdx.x & 63;
-    int nin = big_batch(aj, col, bv, c0, c1, outs, W, P, acc, a);
+    int nin = big_batch(aj, col, bv, c0, c1, outs, W, P, acc, err);
     cj += nin;
     int32_t last = col;  // the batch whose first out-of-window entry ends the piece
     while (nin == 64) {
         int32_t cb[kPieceBatches];
-        float vb[kPieceBatches];
+        Acc vb[kPieceBatches];
 #pragma unroll
         for (int b = 0; b < kPieceBatches; ++b) {
             const int bp = cj + b * 64 + lane;
-            const int2 cv = bp < ej ? pairs[bp] : make_int2(INT32_MAX, 0);
-            cb[b] = cv.x;
-            vb[b] = __int_as_float(cv.y);
+            const Pair cv = bp < ej ? pairs[bp] : pair_none<Pair>();
+            cb[b] = pair_col(cv);
+            pair_value(cv, vb[b]);
         }
 #pragma unroll
         for (int b = 0; b < kPieceBatches; ++b) {
             if (nin == 64) {  // uniform: the previous batch lay wholly in the window
-                nin = big_batch(aj, cb[b], vb[b], c0, c1, outs, W, P, acc, a);
+                nin = big_batch(aj, cb[b], vb[b], c0, c1, outs, W, P, acc, err);
                 cj += nin;
                 last = cb[b];
             }
@@ -373,14 +377,15 @@ __device__ __forceinline__ int2 big_piece(int cj, int ej, float aj, int32_t col,
 // Up to four live A entries of a window, taken from a lane mask in lane
 // (= ascending k) order, with the first 64 entries of each piece loaded
 // from `pairs` (B's pairs, or their column-blocked copy).
+template <class Acc, class Pair>
 struct PieceGroup {
     int n;
     int j[4], c[4], e[4];
-    float av[4], bv[4];
+    Acc av[4], bv[4];
     int32_t col[4];
 
-    __device__ __forceinline__ void take(uint64_t &mask, int4 ce, float a_val,
-                                         const int2 *__restrict__ pairs)
+    __device__ __forceinline__ void take(uint64_t &mask, int4 ce, Acc a_val,
+                                         const Pair *__restrict__ pairs)
     {
         const int lane = threadIdx.x & 63;
         n = 0;
@@ -400,9 +405,9 @@ struct PieceGroup {
             av[u] = lane_bcast(a_val, j[u]);
             const int bp = c[u] + lane;
             const bool ok = u < n && bp < e[u];
-            const int2 cv = ok ? pairs[bp] : make_int2(INT32_MAX, 0);
-            col[u] = cv.x;
-            bv[u] = __int_as_float(cv.y);
+            const Pair cv = ok ? pairs[bp] : pair_none<Pair>();
+            col[u] = pair_col(cv);
+            pair_value(cv, bv[u]);
         }
     }
 };
@@ -410,17 +415,19 @@ struct PieceGroup {
 // COLS: the pass also writes each window's column indices (g.cols_out),
 // expanded from the rank dictionary into the accumulator's LDS once the
 // values are out, so a one-shot needs no separate big-row structure fill.
-template <int NW, int PW, int MAXCH, bool COLS>
-__global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, BigArgs g)
+template <int NW, int PW, int MAXCH, bool COLS, class Args>
+__global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigArgs g)
 {
+    using Acc = typename Args::acc_t;
+    using Pair = typename Args::pair_t;
     constexpr int MAXW = MAXCH * kChunkWords;
-    __shared__ float acc_s[NW][PW];
+    __shared__ Acc acc_s[NW][PW];
     __shared__ uint32_t W_s[NW][MAXW];
     __shared__ uint16_t P_s[NW][MAXW];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t it = (int64_t)blockIdx.x * NW + wv;
     if (it >= nsegs) return;  // whole wave leaves; no block barriers below
-    float *acc = acc_s[wv];
+    Acc *acc = acc_s[wv];
     uint32_t *W = W_s[wv];
     uint16_t *P = P_s[wv];
     const int4 sg = g.segs[it];
@@ -429,7 +436,7 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
     const int as = a.arp[row], ae = a.arp[row + 1];
     const int32_t rlo = big_base(g.lo[row]);
     const int nchunks = big_nchunks(g.lo[row], g.hi[row]);
-    const int2 *__restrict__ pairs = a.bcv;
+    const Pair *__restrict__ pairs = a.bcv;
     const int64_t boff = g.big_off[row];
     const int64_t c32 = boff / kChunkWords;
     const int nwin = g.big_nwin[row];
@@ -511,7 +518,7 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
             P[i] = (uint16_t)run;
             run += __popc(wl[j]);
         }
-        for (int i = lane; i < outs; i += 64) acc[i] = 0.0f;
+        for (int i = lane; i < outs; i += 64) acc[i] = Acc(0);
         wave_sync();
 
         // groups of 64 entries whose smallest next column lies before the
@@ -525,10 +532,10 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
                 groups &= groups - 1;
                 const int ap = base + lane;
                 int4 ce = make_int4(0, 0, INT32_MAX, 0);
-                float av = 0.0f;
+                Acc av = Acc(0);
                 if (ap < ae) {
                     ce = cur_a[ap - as];
-                    av = a.aval[ap];
+                    av = Acc(a.aval[ap]);
                 }
                 // the cursor carries the column it points at: no load to test it
                 const bool live = ce.x < ce.y && ce.z < c1;
@@ -536,13 +543,13 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, NumArgs a, B
                 uint64_t mask = __ballot(live);
                 while (mask) {
                     // up to four live entries: issue all their loads first
-                    PieceGroup g0;
+                    PieceGroup<Acc, Pair> g0;
                     g0.take(mask, ce, av, pairs);
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         if (u < g0.n) {
                             const int2 nc = big_piece(g0.c[u], g0.e[u], g0.av[u], g0.col[u],
-                                                      g0.bv[u], c0, c1, outs, W, P, acc, a,
+                                                      g0.bv[u], c0, c1, outs, W, P, acc, a.err,
                                                       pairs);
                             compiler_fence();
                             if (lane == g0.j[u]) {
@@ -638,17 +645,21 @@ void launch_big_plan(hipStream_t st, const int32_t *rows, int nrows, const BigAr
     if (nrows > 0) k_big_plan<<<(nrows + 255) / 256, 256, 0, st>>>(rows, nrows, g, crp, ctr);
 }
 
-template <int PW, int MAXCH>
-static void launch_num_big_shape(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g)
+template <int PW, int MAXCH, class Args>
+static void launch_num_big_shape(hipStream_t st, int nsegs, const Args &a, const BigArgs &g)
 {
     constexpr int NW = 2;
-    if (g.cols_out)
-        k_num_big<NW, PW, MAXCH, true><<<(nsegs + NW - 1) / NW, NW * 64, 0, st>>>(nsegs, a, g);
-    else
-        k_num_big<NW, PW, MAXCH, false><<<(nsegs + NW - 1) / NW, NW * 64, 0, st>>>(nsegs, a, g);
+    if constexpr (std::is_same<Args, NumArgs>::value) {
+        if (g.cols_out) {
+            k_num_big<NW, PW, MAXCH, true><<<(nsegs + NW - 1) / NW, NW * 64, 0, st>>>(nsegs, a, g);
+            return;
+        }
+    }
+    k_num_big<NW, PW, MAXCH, false><<<(nsegs + NW - 1) / NW, NW * 64, 0, st>>>(nsegs, a, g);
 }
 
-void launch_num_big(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g)
+template <class Args>
+static void launch_num_big_any(hipStream_t st, int nsegs, const Args &a, const BigArgs &g)
 {
     if (nsegs <= 0) return;
     // the shape must be one of kBigShapes (the plan only sets those)
@@ -662,6 +673,21 @@ void launch_num_big(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &
         launch_num_big_shape<1024, 8>(st, nsegs, a, g);
     else
         launch_num_big_shape<2048, 16>(st, nsegs, a, g);
+}
+
+void launch_num_big(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g)
+{
+    launch_num_big_any(st, nsegs, a, g);
+}
+
+void launch_num_big(hipStream_t st, int nsegs, const NumArgsD &a, const BigArgs &g)
+{
+    launch_num_big_any(st, nsegs, a, g);
+}
+
+void launch_num_big(hipStream_t st, int nsegs, const NumArgsI &a, const BigArgs &g)
+{
+    launch_num_big_any(st, nsegs, a, g);
 }
 
 } // namespace rvsp

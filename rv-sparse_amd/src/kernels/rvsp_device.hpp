@@ -82,6 +82,43 @@ __device__ __forceinline__ float lane_bcast(float v, int src)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
 
+__device__ __forceinline__ double lane_bcast(double v, int src)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// ---- value types of the numeric pass ----------------------------------------
+// FP32 (rv-sparse's contract) and the legacy dtypes: FP64 in and out, INT8 in
+// with INT32 accumulation. B is read as (column, value) pairs: int2 for FP32
+// (value bits) and INT8 (value widened to int), PairD for FP64.
+struct __align__(16) PairD {
+    int32_t col, pad;
+    double val;
+};
+
+__device__ __forceinline__ int32_t pair_col(int2 p) { return p.x; }
+__device__ __forceinline__ int32_t pair_col(const PairD &p) { return p.col; }
+template <class Pair>
+__device__ __forceinline__ Pair pair_none();
+template <>
+__device__ __forceinline__ int2 pair_none<int2>() { return make_int2(INT32_MAX, 0); }
+template <>
+__device__ __forceinline__ PairD pair_none<PairD>() { return PairD{INT32_MAX, 0, 0.0}; }
+__device__ __forceinline__ void pair_value(int2 p, float &v) { v = __int_as_float(p.y); }
+__device__ __forceinline__ void pair_value(int2 p, int32_t &v) { v = p.y; }
+__device__ __forceinline__ void pair_value(const PairD &p, double &v) { v = p.val; }
+
+// One product, rounded once (no fused multiply-add); INT32 wraps.
+__device__ __forceinline__ float vmul(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double vmul(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ int32_t vmul(int32_t a, int32_t b)
+{
+    return (int32_t)((uint32_t)a * (uint32_t)b);
+}
+
 // Orders LDS traffic of one wave: everything issued before is visible to
 // every lane of the wave after. Same sequence rocPRIM uses for wave-level
 // cooperation through LDS.

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "rvsp_device.hpp"
+
 namespace rvsp {
 
 // Counters the analysis kernels accumulate; copied to the host once per
@@ -62,17 +64,26 @@ struct SymArgs {
     uint32_t *bms_spill;  // kBmsWords words per one-pass-bitmap row (its slot in the bin), or null
 };
 
-struct NumArgs {
+// Operands of a numeric pass: In = A's and B's value type, Acc = the
+// accumulation and C value type, Pair = B's (column, value) pair.
+template <class In, class Acc, class Pair>
+struct NumArgsT {
+    using in_t = In;
+    using acc_t = Acc;
+    using pair_t = Pair;
     const int32_t *arp, *aci;
-    const float *aval;
+    const In *aval;
     const int32_t *brp, *bci;
-    const float *bval;
-    const int2 *bcv;      // B as (column, value bits) pairs: one 8-byte load per entry
+    const In *bval;
+    const Pair *bcv;      // B as (column, value) pairs: one load per entry
     const int32_t *crp, *cci;
-    float *cval;
+    Acc *cval;
     const int32_t *lo;
     int *err;
 };
+using NumArgs = NumArgsT<float, float, int2>;      // rv-sparse's FP32 contract
+using NumArgsD = NumArgsT<double, double, PairD>;  // legacy FP64
+using NumArgsI = NumArgsT<int8_t, int32_t, int2>;  // legacy INT8 -> INT32
 
 void launch_row_stats(hipStream_t st, int32_t m, const int32_t *arp, const int32_t *aci,
                       const int32_t *brp, const int32_t *bci, int64_t *ops, int32_t *lo,
@@ -109,6 +120,27 @@ void launch_big_fill(hipStream_t st, const BigArgs &g, const int32_t *crp, int32
 void launch_big_plan(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,
                      const int32_t *crp, AnalysisCounters *ctr);
 void launch_num_big(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g);
+void launch_num_big(hipStream_t st, int nsegs, const NumArgsD &a, const BigArgs &g);
+void launch_num_big(hipStream_t st, int nsegs, const NumArgsI &a, const BigArgs &g);
+
+// legacy dtypes (rvsp_legacy.hip): the numeric pass over the non-big rows
+// (`rows`) and the big-row segments, B's pairs for the pass, and the
+// compaction that drops entries equal to zero (counts, scan scratch and
+// the new row pointer in rp_out, columns and values in ci_out / cv_out)
+void launch_numeric_typed(hipStream_t st, const NumArgsD &a, const int32_t *rows, int nrows,
+                          const BigArgs &big, int nsegs);
+void launch_numeric_typed(hipStream_t st, const NumArgsI &a, const int32_t *rows, int nrows,
+                          const BigArgs &big, int nsegs);
+void launch_pairs_typed(hipStream_t st, int32_t nnz, const int32_t *bci, const double *bval,
+                        PairD *out);
+void launch_pairs_typed(hipStream_t st, int32_t nnz, const int32_t *bci, const int8_t *bval,
+                        int2 *out);
+void launch_drop_zeros(hipStream_t st, int32_t m, const int32_t *crp, const int32_t *cci,
+                       const double *cval, int32_t *cnt, int64_t *tsum, int64_t *total,
+                       int32_t *rp_out, int32_t *ci_out, double *cv_out);
+void launch_drop_zeros(hipStream_t st, int32_t m, const int32_t *crp, const int32_t *cci,
+                       const int32_t *cval, int32_t *cnt, int64_t *tsum, int64_t *total,
+                       int32_t *rp_out, int32_t *ci_out, int32_t *cv_out);
 
 // reused structure (rvsp_reuse.hip): per-product output ranks of the rows on
 // the LDS paths; with pmap null only the per-row product offsets are made

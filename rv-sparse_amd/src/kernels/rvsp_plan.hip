@@ -704,6 +704,80 @@ rvsp_status_t rvsp_hip_plan_fill_numeric(rvsp_hip_plan_t *p, const float *aval,
     return RVSP_SUCCESS;
 }
 
+rvsp_status_t rvsp_hip_plan_numeric_typed(rvsp_hip_plan_t *p, rvsp_dtype_t in_dtype,
+                                          const void *aval, const void *bval, const int32_t *cci,
+                                          void *cval)
+{
+    if (!p || !aval || !bval || !cci || !cval) return RVSP_ERROR_NULL_POINTER;
+    if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
+    if (in_dtype != RVSP_DTYPE_FP64 && in_dtype != RVSP_DTYPE_INT8)
+        return RVSP_ERROR_UNSUPPORTED_DTYPE;
+    hipStream_t st = p->stream;
+    PhaseScope timer(st, PH_NUMERIC);
+    // non-empty, non-big rows: the numeric bins before NB_WIN, contiguous in nperm
+    const int32_t *rows = p->nperm + p->num_off[NB_HASH0];
+    const int nrows = p->num_off[NB_WIN] - p->num_off[NB_HASH0];
+    rvsp_status_t s;
+    if (in_dtype == RVSP_DTYPE_FP64) {
+        PairD *pairs = nullptr;
+        if ((s = dalloc((void **)&pairs, (size_t)p->nnz_b * sizeof(PairD), st)) != RVSP_SUCCESS)
+            return s;
+        launch_pairs_typed(st, p->nnz_b, p->bci, (const double *)bval, pairs);
+        const NumArgsD a{p->arp, p->aci, (const double *)aval, p->brp, p->bci,
+                         (const double *)bval, pairs, p->crp, cci, (double *)cval, p->lo, p->err};
+        launch_numeric_typed(st, a, rows, nrows, p->big, p->nseg);
+        dfree(pairs, st);
+    } else {
+        int2 *pairs = nullptr;
+        if ((s = dalloc((void **)&pairs, (size_t)p->nnz_b * sizeof(int2), st)) != RVSP_SUCCESS)
+            return s;
+        launch_pairs_typed(st, p->nnz_b, p->bci, (const int8_t *)bval, pairs);
+        const NumArgsI a{p->arp, p->aci, (const int8_t *)aval, p->brp, p->bci,
+                         (const int8_t *)bval, pairs, p->crp, cci, (int32_t *)cval, p->lo, p->err};
+        launch_numeric_typed(st, a, rows, nrows, p->big, p->nseg);
+        dfree(pairs, st);
+    }
+    HIP_TRY(hipGetLastError());
+    return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_hip_plan_drop_zeros(rvsp_hip_plan_t *p, rvsp_dtype_t value_dtype,
+                                       const int32_t *cci, const void *cval, int32_t *rp_out,
+                                       int32_t *ci_out, void *cv_out, int32_t *nnz_out)
+{
+    if (!p || !cci || !cval || !rp_out || !ci_out || !cv_out || !nnz_out)
+        return RVSP_ERROR_NULL_POINTER;
+    if (!p->analysed) return RVSP_ERROR_INVALID_ARGUMENT;
+    if (value_dtype != RVSP_DTYPE_FP64 && value_dtype != RVSP_DTYPE_INT32)
+        return RVSP_ERROR_UNSUPPORTED_DTYPE;
+    hipStream_t st = p->stream;
+    const size_t M = (size_t)p->m;
+    const int ntiles = scan_tiles(p->m);
+    size_t off = 0;
+    const size_t o_cnt = off; off = align256(off + M * 4);
+    const size_t o_ts = off; off = align256(off + (size_t)ntiles * 8);
+    const size_t o_tot = off; off = align256(off + 8);
+    void *scratch = nullptr;
+    rvsp_status_t s = dalloc(&scratch, off, st);
+    if (s != RVSP_SUCCESS) return s;
+    char *b = (char *)scratch;
+    if (value_dtype == RVSP_DTYPE_FP64)
+        launch_drop_zeros(st, p->m, p->crp, cci, (const double *)cval, (int32_t *)(b + o_cnt),
+                          (int64_t *)(b + o_ts), (int64_t *)(b + o_tot), rp_out, ci_out,
+                          (double *)cv_out);
+    else
+        launch_drop_zeros(st, p->m, p->crp, cci, (const int32_t *)cval, (int32_t *)(b + o_cnt),
+                          (int64_t *)(b + o_ts), (int64_t *)(b + o_tot), rp_out, ci_out,
+                          (int32_t *)cv_out);
+    HIP_TRY(hipGetLastError());
+    int32_t kept = 0;
+    HIP_TRY(hipMemcpyAsync(&kept, rp_out + p->m, 4, hipMemcpyDeviceToHost, st));
+    dfree(scratch, st);
+    HIP_TRY(hipStreamSynchronize(st));
+    *nnz_out = kept;
+    return RVSP_SUCCESS;
+}
+
 rvsp_status_t rvsp_hip_plan_prepare_reuse(rvsp_hip_plan_t *p, const int32_t *cci)
 {
     if (!p || !cci) return RVSP_ERROR_NULL_POINTER;
