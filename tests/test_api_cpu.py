@@ -179,10 +179,15 @@ def test_oneshot_dispatch_rules():
     a, b = A.struct(), B.struct()
     c = rvsp.CsrStruct()
     assert L.rvsp_spgemm_csr(None, C.byref(b), C.byref(c), None) == -1
-    # FP64 / INT8 legacy kernels are not part of this build
-    for be, ind, outd in ((0, rvsp.DTYPE_FP64, rvsp.DTYPE_FP64), (0, rvsp.DTYPE_INT8, rvsp.DTYPE_INT32)):
+    # legacy dtypes over FP32 matrices fail the legacy wrappers' dtype checks
+    # (csr_spgemm_wrappers.c:362 INVALID_ARGUMENT, :43 UNSUPPORTED_DTYPE)
+    for be, ind, outd, want in ((0, rvsp.DTYPE_FP64, rvsp.DTYPE_FP64, rvsp.ERROR_INVALID_ARGUMENT),
+                                (0, rvsp.DTYPE_INT8, rvsp.DTYPE_INT32, rvsp.ERROR_UNSUPPORTED_DTYPE)):
         opt = rvsp.Options(be, ind, outd)
-        assert L.rvsp_spgemm_csr(C.byref(a), C.byref(b), C.byref(c), C.byref(opt)) == -5
+        assert L.rvsp_spgemm_csr(C.byref(a), C.byref(b), C.byref(c), C.byref(opt)) == want
+    # a legacy dtype pair the legacy dispatch does not know
+    opt = rvsp.Options(0, rvsp.DTYPE_FP64, rvsp.DTYPE_INT32)
+    assert L.rvsp_spgemm_csr(C.byref(a), C.byref(b), C.byref(c), C.byref(opt)) == -5
     # RVV reaches set_algo after the input checks, then is unsupported
     opt = rvsp.Options(rvsp.BACKEND_RVV_INTRINSICS, 3, 3)
     assert L.rvsp_spgemm_csr(C.byref(a), C.byref(b), C.byref(c), C.byref(opt)) == -5
