@@ -9,8 +9,11 @@ one-shot kernel. Default workload: R-MAT 2^18 x 2^18 (a,b,c,d = .57,.19,.19,
 .05), 16 x 2^18 edge draws, C = A*A.
 
 Prints one JSON line. Options: --steps K --warmup W --gpus N (1 only)
---workload {rmat,uniform,mixed,stencil,batched} --algo {hip,learned} --no-cpu.
+--workload {rmat,uniform,mixed,stencil,batched,mtx:PATH,gen:R:C:D:SEED}
+--algo {hip,learned} --no-cpu.
   rmat, uniform, mixed  step = one device one-shot product
+  mtx:PATH, gen:...     same, on a local Matrix Market file (A*A) or on
+                        rv-sparse's generated pair (A*B)
   stencil               analysis once; step = one numeric pass with new
                         values (hipGraph replay; per-call launches reported)
   batched               512 uneven products; step = the whole batch in one
@@ -61,7 +64,33 @@ def load_workload(name):
         if A.rows != A.cols:
             raise SystemExit(f"{path}: {A.rows} x {A.cols} (inferred) is not square, no A*A")
         return A, A, True, f"{os.path.basename(path)} ({A.rows} x {A.cols}, nnz {A.nnz}), C=A*A, file values FP32"
+    if name.startswith("gen:"):
+        # rv-sparse's bench --gen R C DENSITY SEED (bench.c:677-707): A = R x C,
+        # B = C x R (seed + 1), from the library's genmat generator
+        R, Cc, D, S = gen_args(name)
+        A, B = rvsp.gen_bench_pair(R, Cc, D, S)
+        return A, B, False, (f"genmat {R} x {Cc} * {Cc} x {R}, density {D}, seed {S} "
+                             f"(nnz {A.nnz} + {B.nnz}), C=A*B, values 0.1..0.9 FP32")
     raise SystemExit(f"unknown workload {name}")
+
+
+def gen_args(name):
+    try:
+        R, Cc, D, S = name[4:].split(":")
+        return int(R), int(Cc), float(D), int(S)
+    except ValueError:
+        raise SystemExit(f"{name}: expected gen:ROWS:COLS:DENSITY:SEED")
+
+
+def default_csv_label(workload):
+    """The label rv-sparse's run_bench.sh gives a case (run_bench.sh:1432
+    for generated inputs: gen_<R>x<C>_d<DENSITY>; a matrix file's stem)."""
+    if workload.startswith("gen:"):
+        R, Cc, _, _ = gen_args(workload)
+        return f"gen_{R}x{Cc}_d{workload.split(':')[3]}"
+    if workload.startswith("mtx:"):
+        return os.path.splitext(os.path.basename(workload[4:]))[0]
+    return workload
 
 
 def device_rows(c: rvsp.CsrStruct, rows):
@@ -235,7 +264,8 @@ def bench_csv(args):
     d.close()
     ops = ops.astype(np.float64)
     mean = float(ops.mean())
-    meta = {"label": args.csv_label, "rows": A.rows, "cols": A.cols, "nnz_a": A.nnz, "nnz_b": B.nnz,
+    label = args.csv_label if args.csv_label is not None else default_csv_label(args.workload)
+    meta = {"label": label, "rows": A.rows, "cols": A.cols, "nnz_a": A.nnz, "nnz_b": B.nnz,
             "flops": 2.0 * float(ops.sum()), "op_mean": mean, "op_max": float(ops.max()),
             "op_var": float(((ops - mean) ** 2).mean())}
     try:
@@ -293,10 +323,24 @@ def bench_csv(args):
         times.setdefault("scalar_f32", []).append(dt)
     with open(args.csv, "w") as f:
         f.write("\n".join(out) + "\n")
-    print(json.dumps({"csv": args.csv, "workload": desc, "rows": len(out) - 1,
+    # rv-sparse's analyze.py statistics over the rows just written: speedup
+    # over the scalar kernel (arm "baseline") with its bootstrap interval
+    from rvsp import report
+    summary = report.summarize(report.load_rows(args.csv))
+    report.attach_speedups(summary)
+    log("\n".join(report.table(summary)))
+
+    def num(x):
+        return None if x is None or x != x else round(float(x), 4)
+
+    print(json.dumps({"csv": args.csv, "workload": desc, "label": label, "rows": len(out) - 1,
                       "median_s": {k: round(float(np.median(v)), 6) for k, v in times.items()},
                       "median_gflops": {k: round(meta["flops"] / float(np.median(v)) / 1e9, 3)
-                                        for k, v in times.items()}}), flush=True)
+                                        for k, v in times.items()},
+                      "speedup_vs_scalar": {key[3]: {"x": num(e["speedup"]),
+                                                     "ci95": [num(e["speedup_lo"]), num(e["speedup_hi"])],
+                                                     "status": e["status"]}
+                                            for key, e in summary.items()}}), flush=True)
 
 
 def main():
@@ -305,7 +349,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rmat",
-                    help="rmat, mixed, uniform, stencil, batched, or mtx:PATH (a local .mtx file, C=A*A)")
+                    help="rmat, mixed, uniform, stencil, batched, mtx:PATH (a local .mtx file, C=A*A), or "
+                         "gen:ROWS:COLS:DENSITY:SEED (rv-sparse's generated pair, C=A*B)")
     ap.add_argument("--algo", default="learned", choices=["hip", "learned"],
                     help="learned: MLP row selectors (the plain heuristic is timed beside it)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -318,7 +363,8 @@ def main():
     ap.add_argument("--csv", default=None,
                     help="instead of the JSON bench line: write rv-sparse bench CSV rows (one per "
                          "timed run) for hip_f32, hip_f32_dev and scalar_f32 to this file")
-    ap.add_argument("--csv-label", default="-")
+    ap.add_argument("--csv-label", default=None,
+                    help="the CSV label column (default: rv-sparse's label for the workload)")
     ap.add_argument("--csv-cpu-runs", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -34,5 +34,11 @@ gcc -std=c11 -O2 -fPIC -shared -w -I"$REF/tools/include" \
     "$REF/tools/src/mtx_to_csr_formatter.c" "$REF/tools/src/vec.c" \
     -o "$OUT/librvsp_mtx_ref.so.tmp"
 mv "$OUT/librvsp_mtx_ref.so.tmp" "$OUT/librvsp_mtx_ref.so"
+# rv-sparse's synthetic generator (tools/src/genmat.c, single-threaded: the
+# OpenMP pragmas are ignored without -fopenmp, as in its default build), the
+# checker of genmat_generate_csr
+gcc -std=c11 -O3 -fPIC -shared -w -I"$REF/tools/include" \
+    "$REF/tools/src/genmat.c" -o "$OUT/librvsp_genmat_ref.so.tmp" -lm
+mv "$OUT/librvsp_genmat_ref.so.tmp" "$OUT/librvsp_genmat_ref.so"
 rm -rf "$OUT/obj"
-echo "build_ref: wrote $OUT/librvsp_ref.so and $OUT/librvsp_mtx_ref.so"
+echo "build_ref: wrote $OUT/librvsp_ref.so, librvsp_mtx_ref.so and librvsp_genmat_ref.so"

@@ -64,6 +64,20 @@ class Options(C.Structure):
     _fields_ = [("backend", C.c_int), ("input_dtype", C.c_int), ("output_dtype", C.c_int)]
 
 
+class GenmatParams(C.Structure):
+    """genmat_params_t (include/genmat.h)."""
+    _fields_ = [("row_cnt", C.c_int), ("col_cnt", C.c_int), ("density", C.c_double),
+                ("cv", C.c_double), ("min", C.c_int), ("imbalance", C.c_double),
+                ("is_symmetric", C.c_int), ("is_column", C.c_int), ("low_bandwidth", C.c_int),
+                ("up_bandwidth", C.c_int), ("random_seed", C.c_int)]
+
+
+class GenmatCsr(C.Structure):
+    """csr_matrix_t (include/genmat.h): int indices, double values."""
+    _fields_ = [("nrows", C.c_int), ("ncols", C.c_int), ("nnz", C.c_longlong),
+                ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("values", C.c_void_p)]
+
+
 _lib = None
 _gen = None
 
@@ -78,6 +92,11 @@ def _declare(lib):
         "rvsp_csr_validate": (C.c_int, [CSRP]),
         "rvsp_csr_destroy": (None, [CSRP]),
         "rvsp_mtx_load": (C.c_int, [C.c_char_p, CSRP]),
+        "genmat_default_params": (GenmatParams, [C.c_int, C.c_int]),
+        "genmat_generate_csr": (GenmatCsr, [GenmatParams]),
+        "genmat_free_csr": (None, [P(GenmatCsr)]),
+        "genmat_save_csr_to_mtx": (C.c_int, [P(GenmatCsr), C.c_char_p]),
+        "rvsp_genmat_bench_pair": (C.c_int, [I32, I32, C.c_double, I32, C.c_int, CSRP, CSRP]),
         "rvsp_spgemm_csr": (C.c_int, [CSRP, CSRP, CSRP, P(Options)]),
         "rvsp_spgemm_descr_create": (C.c_int, [P(V)]),
         "rvsp_spgemm_descr_destroy": (None, [V]),
@@ -418,6 +437,61 @@ def mtx_load(path: str, raise_on_error=True):
     out = Csr(c.rows, c.cols, rp, ci, va)
     lib().rvsp_csr_destroy(C.byref(c))
     return out if raise_on_error else (SUCCESS, out)
+
+
+def genmat_params(rows, cols, **kw) -> GenmatParams:
+    """genmat_default_params(rows, cols) with the named fields replaced
+    (density, cv, min, imbalance, is_symmetric, is_column, low_bandwidth,
+    up_bandwidth, random_seed)."""
+    p = lib().genmat_default_params(rows, cols)
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise TypeError(f"genmat_params_t has no field {k}")
+        setattr(p, k, v)
+    return p
+
+
+def genmat_arrays(m: GenmatCsr):
+    """(rows, cols, row_ptr int32, col_idx int32, values float64) of a csr_matrix_t."""
+    n = int(m.nnz)
+
+    def host(ptr, ct, count, dt):
+        if not count:
+            return np.zeros(0, dt)
+        return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(count,)).copy()
+
+    return (m.nrows, m.ncols, host(m.row_ptr, C.c_int32, m.nrows + 1, np.int32),
+            host(m.col_idx, C.c_int32, n, np.int32), host(m.values, C.c_double, n, np.float64))
+
+
+def genmat(rows, cols, **kw):
+    """genmat_generate_csr: (rows, cols, row_ptr, col_idx, values float64), or
+    None when the generator reports an error (nrows == -1)."""
+    m = lib().genmat_generate_csr(genmat_params(rows, cols, **kw))
+    if m.nrows < 0:
+        return None
+    out = genmat_arrays(m)
+    lib().genmat_free_csr(C.byref(m))
+    return out
+
+
+def gen_bench_pair(rows, cols, density, seed):
+    """rv-sparse's bench inputs for `--gen R C DENSITY SEED` in FP32: host
+    Csr A (rows x cols, seed) and B (cols x rows, seed + 1)."""
+    a, b = CsrStruct(), CsrStruct()
+    check("genmat_bench_pair", lib().rvsp_genmat_bench_pair(rows, cols, density, seed, DTYPE_FP32,
+                                                            C.byref(a), C.byref(b)))
+    out = []
+    for c in (a, b):
+        n = c.nnz
+        rp = np.ctypeslib.as_array(C.cast(c.row_ptr, C.POINTER(C.c_int32)), shape=(c.rows + 1,)).copy()
+        ci = (np.ctypeslib.as_array(C.cast(c.col_idx, C.POINTER(C.c_int32)), shape=(n,)).copy()
+              if n else np.zeros(0, np.int32))
+        va = (np.ctypeslib.as_array(C.cast(c.values, C.POINTER(C.c_float)), shape=(n,)).copy()
+              if n else np.zeros(0, np.float32))
+        out.append(Csr(c.rows, c.cols, rp, ci, va))
+        lib().rvsp_csr_destroy(C.byref(c))
+    return out[0], out[1]
 
 
 def device_csr_to_host(c: CsrStruct) -> Csr:
