@@ -31,10 +31,15 @@ namespace rvsp {
 // issued before any of their adds, so up to eight loads per lane are in
 // flight; the adds of each k-step are then issued in k order, separated by
 // compiler fences (B is read-only, so hoisting its loads is safe); B is read
-// as (column, value) pairs, one 8-byte load per entry. The
+// as (column, value) pairs, one 8-byte load per entry. A B_k longer than G
+// entries is finished kTailBatches chunks at a time, their loads issued
+// together, so a long B row costs one round trip per kTailBatches * G
+// entries rather than per G. The
 // entry of a k-step reaches the group's lanes through a scalar register
 // (G = 64: one row per wave) or through the group's LDS table M of G
 // entries (one 16-byte load per k-step instead of lane permutes).
+constexpr int kTailBatches = 4;
+
 template <int G, typename Add>
 __device__ __forceinline__ void walk_row_ordered(const NumArgs &a, int as, int ae, KMeta *M,
                                                  Add add)
@@ -82,9 +87,18 @@ __device__ __forceinline__ void walk_row_ordered(const NumArgs &a, int as, int a
             for (int u = 0; u < 4; ++u) {
                 if (j0 + u < nch) {  // uniform within the group
                     if (cu[u] >= 0) add(cu[u], __fmul_rn(au[u], vu[u]));
-                    for (int bp = su[u] + G + lane; bp < eu[u]; bp += G) {
-                        const int2 cv = a.bcv[bp];
-                        add(cv.x, __fmul_rn(au[u], __int_as_float(cv.y)));
+                    // the rest of B_k, kTailBatches chunks of G entries per
+                    // round: one k, distinct columns, so any add order works
+                    for (int bp0 = su[u] + G; bp0 < eu[u]; bp0 += kTailBatches * G) {
+                        int2 tv[kTailBatches];
+#pragma unroll
+                        for (int t = 0; t < kTailBatches; ++t) {
+                            const int bp = bp0 + t * G + lane;
+                            tv[t] = bp < eu[u] ? a.bcv[bp] : make_int2(-1, 0);
+                        }
+#pragma unroll
+                        for (int t = 0; t < kTailBatches; ++t)
+                            if (tv[t].x >= 0) add(tv[t].x, __fmul_rn(au[u], __int_as_float(tv[t].y)));
                     }
                     compiler_fence();
                 }

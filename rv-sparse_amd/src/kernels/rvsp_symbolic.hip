@@ -132,7 +132,8 @@ __global__ __launch_bounds__(256) void k_bin_scatter(int32_t m, const uint8_t *_
 // lanes take the row's A entries as + off + step*t (t = 0, 1, ...) W at a
 // time, load their B-row bounds together, then issue the first W-lane batch
 // of eight entries' B rows before using any of them, so a row costs about
-// 2 + entries/8 dependent loads instead of 3 per entry.
+// 2 + entries/8 dependent loads instead of 3 per entry. The rest of a B row
+// longer than W goes four W-lane batches per round.
 // ---------------------------------------------------------------------------
 template <int W>
 __device__ __forceinline__ int group_bcast(int v, int j)
@@ -174,8 +175,18 @@ __device__ __forceinline__ void for_each_product_col(const int32_t *__restrict__
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 if (cu[u] >= 0) fn(cu[u]);
-                if (j0 + u < nch)
-                    for (int bp = su[u] + W + lane; bp < eu[u]; bp += W) fn(bci[bp]);
+                if (j0 + u < nch)  // the rest of B_k, four chunks of W per round
+                    for (int bp0 = su[u] + W; bp0 < eu[u]; bp0 += 4 * W) {
+                        int32_t t4[4];
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const int bp = bp0 + t * W + lane;
+                            t4[t] = bp < eu[u] ? bci[bp] : -1;
+                        }
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            if (t4[t] >= 0) fn(t4[t]);
+                    }
             }
         }
     }
