@@ -195,7 +195,8 @@ __device__ __forceinline__ void for_each_product_col(const int32_t *__restrict__
 // ---------------------------------------------------------------------------
 // Symbolic pass, hash variant: G lanes per row, LDS key table of H slots.
 // Count mode writes nnz(C_i); fill mode writes C_i's columns ascending
-// (compaction, then a bitonic sort of H/2 keys padded with INT32_MAX).
+// (compaction, then a bitonic sort padded with INT32_MAX, over the block's
+// largest row count rounded up to a power of two, at most H/2).
 // Product order does not matter for the structure, so lanes take B entries
 // freely.
 // ---------------------------------------------------------------------------
@@ -250,11 +251,17 @@ __global__ __launch_bounds__(256) void k_sym_hash(const int32_t *__restrict__ ro
             }
         }
         __syncthreads();
-        // bitonic sort of P keys, ascending; every group runs the same
-        // stage sequence, so block barriers are uniform.
-        for (int kk = 2; kk <= P; kk <<= 1) {
+        // bitonic sort, ascending, of the first Q slots: Q = the block's
+        // largest row count rounded up to a power of two (the table is
+        // sized from the product count, an upper bound, so Q is often far
+        // below P); the padding past a row's keys is INT32_MAX. Every
+        // thread computes the same Q, so block barriers are uniform.
+        int q = 2;
+        for (int r = 0; r < R; ++r) q = max(q, counts[r]);
+        q = min(P, next_pow2(q));
+        for (int kk = 2; kk <= q; kk <<= 1) {
             for (int j = kk >> 1; j > 0; j >>= 1) {
-                for (int i = lane; i < P; i += G) {
+                for (int i = lane; i < q; i += G) {
                     const int ixj = i ^ j;
                     if (ixj > i) {
                         const int32_t x = L[i], y = L[ixj];
