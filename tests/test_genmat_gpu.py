@@ -41,6 +41,21 @@ def test_generated_pairs_match_oracle(R, Cc, D, S):
         assert_bitexact(Cg, pyoracle.ref_spgemm(A, B), "rv-sparse scalar")
 
 
+# larger generated pairs: log-normal degrees put whole rows of C at full
+# width (16384, 32768 columns), so they cross the dense and big-row paths
+LARGE_CASES = [(16384, 16384, 0.001, 7), (32768, 32768, 0.0005, 9), (8192, 40000, 0.0008, 3)]
+
+
+@pytest.mark.parametrize("R,Cc,D,S", LARGE_CASES)
+def test_large_generated_pairs_match_oracle(R, Cc, D, S):
+    print(f"gen large {R}x{Cc} d{D} seed {S}", flush=True)
+    A, B = rvsp.gen_bench_pair(R, Cc, D, S)
+    dA, dB = rvsp.DeviceCsr(A), rvsp.DeviceCsr(B)
+    want = pyoracle.spgemm(A, B)
+    for algo in (rvsp.ALGO_HIP, rvsp.ALGO_HIP_LEARNED):
+        assert_bitexact(rvsp.spgemm_device(dA, dB, algo=algo), want, f"oracle algo {algo}")
+
+
 def legacy_product(lib, a, b, din, dout):
     c = rvsp.CsrStruct()
     opt = rvsp.Options(rvsp.BACKEND_SCALAR, din, dout)
