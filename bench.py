@@ -484,6 +484,7 @@ def main():
             "algorithmic_bytes": alg_bytes,
             "numeric_ms": round(num_avg_ms, 4),
         },
+        "beside_roofline": beside_roofline(products, traffic, num_avg_ms),
         "phases_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in phases.items()},
         "step_roofline_frac": round(step_gbps / HBM_PEAK_GBPS, 4),
         "cpu_baseline": cpu,
@@ -514,6 +515,16 @@ def base_record(args, value, ms_step, config):
         "data": "synthetic (seeded project generator)",
         "config": config,
     }
+
+
+def beside_roofline(products, traffic, numeric_ms):
+    """The two figures reported next to the roofline, not as it: the
+    per-product gather bound (8 bytes of B per product, which L2 / MALL
+    reuse can push past the peak) and the measured HBM traffic's rate."""
+    t = numeric_ms * 1e-3
+    return {"gather_bytes": 8 * int(products),
+            "gather_GBps": round(8.0 * products / t / 1e9, 2) if t > 0 else None,
+            "measured_GBps": round(traffic / t / 1e9, 2) if traffic and t > 0 else None}
 
 
 def bench_reuse(args, n_sets=16):
@@ -582,6 +593,7 @@ def bench_reuse(args, n_sets=16):
                        "frac": round(achieved / HBM_PEAK_GBPS, 4),
                        "traffic": measured_traffic(args.workload),
                        "algorithmic_bytes": alg_bytes, "numeric_ms": round(num_graph, 4)}
+    rec["beside_roofline"] = beside_roofline(products, rec["roofline"]["traffic"], num_graph)
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline(A, A)
     print(json.dumps(rec), flush=True)
 
@@ -653,6 +665,7 @@ def bench_batched(args):
                        "frac": round(achieved / HBM_PEAK_GBPS, 4),
                        "traffic": measured_traffic(args.workload),
                        "algorithmic_bytes": alg_bytes, "numeric_ms": round(num_ms, 4)}
+    rec["beside_roofline"] = beside_roofline(products, rec["roofline"]["traffic"], num_ms)
     rec["phases_ms_per_batch"] = {k: round(v[0] / max(args.steps, 1), 4) for k, v in ph.items()}
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline_batch(pairs)
     print(json.dumps(rec), flush=True)
