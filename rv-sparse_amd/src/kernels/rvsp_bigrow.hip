@@ -44,6 +44,18 @@ namespace rvsp {
 constexpr int kSymLong = 48;    // B rows longer than this are shared by a wave
 constexpr int kSymPiece = 512;  // entries per wave piece (8 per lane)
 
+// LDS slot of bitmap word w: its low six bits XORed with a multiplicative
+// hash of the rest. Skewed column distributions (R-MAT's bits are each 0
+// with probability a + c) crowd the words that share their low bits, and
+// so one LDS bank; the hash spreads them. A bijection on every 64-word
+// block, so the bitmap needs its word count rounded up to 64.
+static_assert(kBigPassWords % 64 == 0, "a full bitmap pass is whole 64-word blocks");
+
+__device__ __forceinline__ int lds_word(int w)
+{
+    return w ^ (int)(((uint32_t)(w >> 6) * 2654435761u) >> 26);
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows, int32_t nrows,
                                                 BigArgs g)
@@ -70,7 +82,7 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
         const int pwn = min(passw, nwords - pw0);
         const int32_t base_col = blo + pw0 * 32;
         const int32_t pcols = pwn * 32;
-        for (int w = tid; w < pwn; w += NT) bits[w] = 0u;
+        for (int w = tid; w < pwn; w += NT) bits[lds_word(w)] = 0u;
         __syncthreads();
         for (int cb = as; cb < ae; cb += NT) {
             const int myap = cb + tid;
@@ -96,7 +108,8 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
                     for (int u = 0; u < 4; ++u) c[u] = bp + u < mbe ? g.bci[bp + u] - base_col : -1;
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
-                        if (c[u] >= 0 && c[u] < pcols) atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
+                        if (c[u] >= 0 && c[u] < pcols)
+                            atomicOr(&bits[lds_word(c[u] >> 5)], 1u << (c[u] & 31));
                 }
             }
             for (int pc = wv; pc < (int)npieces; pc += NWV) {
@@ -119,17 +132,18 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
                 }
 #pragma unroll
                 for (int u = 0; u < kSymPiece / 64; ++u)
-                    if (c[u] >= 0 && c[u] < pcols) atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
+                    if (c[u] >= 0 && c[u] < pcols)
+                        atomicOr(&bits[lds_word(c[u] >> 5)], 1u << (c[u] & 31));
             }
             __syncthreads();
         }
-        for (int w = tid; w < pwn; w += NT) g.gbits[off + pw0 + w] = bits[w];
+        for (int w = tid; w < pwn; w += NT) g.gbits[off + pw0 + w] = bits[lds_word(w)];
         const int nch = pwn / kChunkWords;
         int part = 0;
         for (int ch = tid; ch < nch; ch += NT) {
             int sc = 0;
 #pragma unroll 8
-            for (int w = 0; w < kChunkWords; ++w) sc += __popc(bits[ch * kChunkWords + w]);
+            for (int w = 0; w < kChunkWords; ++w) sc += __popc(bits[lds_word(ch * kChunkWords + w)]);
             g.gchunk[off / kChunkWords + pw0 / kChunkWords + ch] = sc;
             part += sc;
         }
@@ -599,7 +613,9 @@ static void launch_sym_big_nt(hipStream_t st, const int32_t *rows, int nrows, co
                                   hipFuncAttributeMaxDynamicSharedMemorySize, kBigPassWords * 4);
         raised = true;
     }
-    k_sym_big<NT><<<nrows, NT, (size_t)g.pass_words * 4, st>>>(rows, nrows, g);
+    // whole 64-word blocks for the LDS slot hash (kBigPassWords is one)
+    const size_t lds_words = ((size_t)g.pass_words + 63) & ~(size_t)63;
+    k_sym_big<NT><<<nrows, NT, lds_words * 4, st>>>(rows, nrows, g);
 }
 
 void launch_sym_big(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g)
