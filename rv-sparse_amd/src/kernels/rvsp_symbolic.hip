@@ -31,6 +31,40 @@ __device__ __forceinline__ int sym_bin_of(int64_t ops, int32_t lo, int32_t hi)
     return SB_BIG;
 }
 
+// Sums |B_k|, and takes the smallest first and largest last column of the
+// B_k, over the A entries ap0 + t * step (t < U, ap0 + t * step < end): the
+// three dependent loads (A column, B row bounds, B end columns) are each
+// issued for all U entries before any is used.
+template <int U>
+__device__ __forceinline__ void row_stats_step(const int32_t *__restrict__ aci,
+                                               const int32_t *__restrict__ brp,
+                                               const int32_t *__restrict__ bci, int ap0, int step,
+                                               int end, int64_t &ops, int32_t &lo, int32_t &hi)
+{
+    int k[U], bs[U], be[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) k[u] = ap0 + u * step < end ? aci[ap0 + u * step] : -1;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        bs[u] = k[u] >= 0 ? brp[k[u]] : 0;
+        be[u] = k[u] >= 0 ? brp[k[u] + 1] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        ops += be[u] - bs[u];
+        if (be[u] > bs[u]) {
+            lo = min(lo, bci[bs[u]]);
+            hi = max(hi, bci[be[u] - 1]);
+        }
+    }
+}
+
+// Per-row product count, product column span and symbolic bin. Four lanes
+// per row (sixteen rows per wave) take the first 32 entries of every row in
+// one step; the entries past that of the wave's longer rows are then taken
+// one row at a time by the whole wave, eight entries per lane per step.
+constexpr int kStatsRowsPerBlock = 64;
+
 __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__restrict__ arp,
                                                    const int32_t *__restrict__ aci,
                                                    const int32_t *__restrict__ brp,
@@ -41,39 +75,42 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
                                                    uint8_t *__restrict__ sbin_out,
                                                    AnalysisCounters *__restrict__ ctr)
 {
-    constexpr int G = 16;
+    constexpr int G = 4, U = 8;
     __shared__ int hist[SB_COUNT];
     __shared__ unsigned long long bprod, bcap;  // block totals: one global atomic each
     if (threadIdx.x < SB_COUNT) hist[threadIdx.x] = 0;
     if (threadIdx.x == 0) bprod = bcap = 0ull;
     __syncthreads();
-    const int lane = threadIdx.x % G;
-    const int64_t row = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+    const int lane = threadIdx.x & 63, gl = lane % G;
+    const int64_t row = (int64_t)blockIdx.x * kStatsRowsPerBlock + threadIdx.x / G;
     int64_t ops = 0;
     int32_t lo = INT32_MAX, hi = -1;
+    int as = 0, ae = 0;
     if (row < m) {
-        // eight entries per lane per step: each of the three dependent loads
-        // (A column, B row bounds, B end columns) is issued for all eight
-        // before any is used, so a long row costs 3 round trips per 8 G entries
-        constexpr int U = 8;
-        const int as = arp[row], ae = arp[row + 1];
-        for (int ap0 = as + lane; ap0 < ae; ap0 += U * G) {
-            int k[U], bs[U], be[U];
+        as = arp[row];
+        ae = arp[row + 1];
+    }
+    const int e1 = min(ae, as + G * U);
+    row_stats_step<U>(aci, brp, bci, as + gl, G, e1, ops, lo, hi);
+    uint64_t longs = __ballot(gl == 0 && ae > e1);
+    while (longs) {
+        const int src = __ffsll((unsigned long long)longs) - 1;
+        longs &= longs - 1;
+        const int ls = lane_bcast(e1, src), le = lane_bcast(ae, src);
+        int64_t o2 = 0;
+        int32_t lo2 = INT32_MAX, hi2 = -1;
+        for (int ap0 = ls + lane; ap0 < le; ap0 += U * 64)
+            row_stats_step<U>(aci, brp, bci, ap0, 64, le, o2, lo2, hi2);
 #pragma unroll
-            for (int u = 0; u < U; ++u) k[u] = ap0 + u * G < ae ? aci[ap0 + u * G] : -1;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                bs[u] = k[u] >= 0 ? brp[k[u]] : 0;
-                be[u] = k[u] >= 0 ? brp[k[u] + 1] : 0;
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                ops += be[u] - bs[u];
-                if (be[u] > bs[u]) {
-                    lo = min(lo, bci[bs[u]]);
-                    hi = max(hi, bci[be[u] - 1]);
-                }
-            }
+        for (int off = 32; off > 0; off >>= 1) {
+            o2 += __shfl_xor(o2, off, 64);
+            lo2 = min(lo2, __shfl_xor(lo2, off, 64));
+            hi2 = max(hi2, __shfl_xor(hi2, off, 64));
+        }
+        if (lane == src) {
+            ops += o2;
+            lo = min(lo, lo2);
+            hi = max(hi, hi2);
         }
     }
 #pragma unroll
@@ -82,7 +119,7 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
         lo = min(lo, __shfl_xor(lo, off, G));
         hi = max(hi, __shfl_xor(hi, off, G));
     }
-    if (row < m && lane == 0) {
+    if (row < m && gl == 0) {
         const int b = sym_bin_of(ops, lo, hi);
         ops_out[row] = ops;
         lo_out[row] = ops ? lo : 0;
@@ -660,7 +697,7 @@ void launch_row_stats(hipStream_t st, int32_t m, const int32_t *arp, const int32
                       const int32_t *brp, const int32_t *bci, int64_t *ops, int32_t *lo,
                       int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr)
 {
-    const int grid = (int)(((int64_t)m + 15) / 16);
+    const int grid = (int)(((int64_t)m + kStatsRowsPerBlock - 1) / kStatsRowsPerBlock);
     k_row_stats<<<grid, 256, 0, st>>>(m, arp, aci, brp, bci, ops, lo, hi, sbin, ctr);
 }
 
