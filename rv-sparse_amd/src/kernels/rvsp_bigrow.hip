@@ -252,18 +252,26 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
     int nwin = 0;
     if (valid) {
         int out = 0, acc = 0, start = 0;
-        // pass 1: windows
-        for (int c = 0; c < nchunks; ++c) {
-            const int cc = g.gchunk[c32 + c];
-            if (c > start && (acc + cc > g.win_outs || c - start >= g.win_chunks)) {
-                g.win_chunk0[c32 + nwin] = start;
-                g.win_out[c32 + nwin] = out;
-                ++nwin;
-                out += acc;
-                start = c;
-                acc = 0;
+        // pass 1: windows (chunk counts read eight per round trip)
+        for (int c0 = 0; c0 < nchunks; c0 += 8) {
+            int cc8[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) cc8[u] = c0 + u < nchunks ? g.gchunk[c32 + c0 + u] : 0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int c = c0 + u;
+                if (c >= nchunks) break;
+                const int cc = cc8[u];
+                if (c > start && (acc + cc > g.win_outs || c - start >= g.win_chunks)) {
+                    g.win_chunk0[c32 + nwin] = start;
+                    g.win_out[c32 + nwin] = out;
+                    ++nwin;
+                    out += acc;
+                    start = c;
+                    acc = 0;
+                }
+                acc += cc;
             }
-            acc += cc;
         }
         g.win_chunk0[c32 + nwin] = start;
         g.win_out[c32 + nwin] = out;
@@ -290,17 +298,28 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
         }
         int s = 0, w0 = 0;
         double est = 0.0;
-        for (int w = 0; w < nwin; ++w) {
-            const int o1 = w + 1 < nwin ? g.win_out[c32 + w + 1] : nz;
-            est += (o1 - g.win_out[c32 + w]) * ratio;
-            if (est >= (double)kSegProducts || w == nwin - 1) {
-                if (pass == 1) {
-                    g.segs[sbase + s] = make_int4(row, w0, w + 1, 0);
-                    g.seg_cur[sbase + s] = cbase + (long long)s * alen;
+        // window output offsets read eight per round trip (plus the next one)
+        for (int wb = 0; wb < nwin; wb += 8) {
+            int o8[9];
+#pragma unroll
+            for (int u = 0; u < 9; ++u) {
+                const int w = wb + u;
+                o8[u] = w < nwin ? g.win_out[c32 + w] : nz;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int w = wb + u;
+                if (w >= nwin) break;
+                est += (o8[u + 1] - o8[u]) * ratio;
+                if (est >= (double)kSegProducts || w == nwin - 1) {
+                    if (pass == 1) {
+                        g.segs[sbase + s] = make_int4(row, w0, w + 1, 0);
+                        g.seg_cur[sbase + s] = cbase + (long long)s * alen;
+                    }
+                    ++s;
+                    w0 = w + 1;
+                    est = 0.0;
                 }
-                ++s;
-                w0 = w + 1;
-                est = 0.0;
             }
         }
         nseg = s;
