@@ -405,6 +405,7 @@ __global__ __launch_bounds__(256) void k_sym_bitmap(const int32_t *__restrict__ 
                                                     const int32_t *__restrict__ brp,
                                                     const int32_t *__restrict__ bci,
                                                     const int32_t *__restrict__ lo_col,
+                                                    const int32_t *__restrict__ hi_col,
                                                     const int64_t *__restrict__ ops,
                                                     uint32_t *__restrict__ spill,
                                                     int32_t *__restrict__ cnt,
@@ -423,13 +424,16 @@ __global__ __launch_bounds__(256) void k_sym_bitmap(const int32_t *__restrict__ 
     const int row = valid ? rows[ridx] : 0;
     uint32_t *Bm = bits + g * WORDS;
     const int32_t base = valid ? lo_col[row] : 0;
+    // words the row's product span covers (the rest stay zero)
+    const int nw = valid ? min(WORDS, (int)(((int64_t)hi_col[row] - base + 32) >> 5)) : 0;
     // a row with many products per bitmap word keeps its count-pass bitmap
     // in HBM (slot ridx of the bin's list, the same in both passes), so the
-    // fill pass reads it back instead of walking the products again
+    // fill pass reads it back instead of walking the products again; only
+    // the span's words travel
     const bool spilled = spill && valid && ops[row] >= kBmsSpillOps;
     uint32_t *slot = spilled ? spill + ridx * WORDS : nullptr;
     if (FILL && spilled) {
-        for (int w = lane; w < WORDS; w += G) Bm[w] = slot[w];
+        for (int w = lane; w < WORDS; w += G) Bm[w] = w < nw ? slot[w] : 0u;
     } else {
         for (int w = lane; w < WORDS; w += G) Bm[w] = 0u;
     }
@@ -442,7 +446,7 @@ __global__ __launch_bounds__(256) void k_sym_bitmap(const int32_t *__restrict__ 
     }
     __syncthreads();
     if (!FILL && spilled)
-        for (int w = lane; w < WORDS; w += G) slot[w] = Bm[w];
+        for (int w = lane; w < nw; w += G) slot[w] = Bm[w];
     if (!valid) return;  // no block barriers below: one row per wave
     if constexpr (FILL) {
         // ascending columns, 64 words per round (lane l: word 64r + l): a
@@ -450,8 +454,9 @@ __global__ __launch_bounds__(256) void k_sym_bitmap(const int32_t *__restrict__ 
         // store instruction covers a short run of C_i and no lane expands
         // more than one word per round
         int run = crp[row];
+        const int rounds = (nw + 63) / 64;  // words past the span are zero
 #pragma unroll 1
-        for (int r = 0; r < WPT; ++r) {
+        for (int r = 0; r < rounds; ++r) {
             uint32_t x = Bm[r * 64 + lane];
             const int c = __popc(x);
             int incl = c;
@@ -678,8 +683,8 @@ static void launch_sym_bins(hipStream_t st, const SymArgs &a, const int32_t *per
         case SB_BMS: {
             const int grid = (nr + 3) / 4;
             k_sym_bitmap<64, kBmsWords, FILL><<<grid, 256, 0, st>>>(
-                rows, nr, a.arp, a.aci, a.brp, a.bci, a.lo, a.ops, a.bms_spill, a.cnt, a.crp,
-                a.cci);
+                rows, nr, a.arp, a.aci, a.brp, a.bci, a.lo, a.hi, a.ops, a.bms_spill, a.cnt,
+                a.crp, a.cci);
             break;
         }
         case SB_BIG:
