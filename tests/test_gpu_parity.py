@@ -45,9 +45,13 @@ def test_fixture_every_strategy(pair, algo):
     say(f"fixture {pair['name']} algo {algo}")
     A, B, Cw = csr_of(pair["a"]), csr_of(pair["b"]), csr_of(pair["c"])
     d = rvsp.Descr()
-    if d.set_algo(algo) == rvsp.ERROR_UNSUPPORTED_BACKEND:
+    if algo in (rvsp.ALGO_RVV, rvsp.ALGO_CONTIG, rvsp.ALGO_ADAPTIVE):
+        # the RISC-V Vector strategies: refused, as by an rv-sparse build without V
+        # (rv-sparse's test_spgemm_reference_f32.c skips them on that status)
+        assert d.set_algo(algo) == rvsp.ERROR_UNSUPPORTED_BACKEND
         d.close()
-        pytest.skip("strategy not provided by this build")
+        return
+    assert d.set_algo(algo) == rvsp.SUCCESS
     a, b = A.struct(), B.struct()
     st, ws, nnz = d.estimate(a, b)
     assert st == 0 and nnz == Cw.nnz
