@@ -280,12 +280,40 @@ __global__ __launch_bounds__(R *G) void k_num_rankw(const int32_t *__restrict__ 
     for (int q = lane; q < nz; q += G) a.cval[cs + q] = S[q];
 }
 
+// Rows per block for a kernel needing `rowb` LDS bytes per row: the count
+// (whole waves, at most 256 threads) that keeps the most rows resident in a
+// CU (its 160 KB of LDS, and at most 32 waves), among blocks of at most
+// 64 KB (the larger count on a tie); rows too big for that take the
+// smallest count that fits a CU.
+constexpr int kLdsPerCu = 163840;
+constexpr int kWavesPerCu = 32;
+
+template <int G>
+constexpr int rows_for_lds(int rowb)
+{
+    for (int pass = 0; pass < 2; ++pass) {
+        const int cap = pass == 0 ? 65536 : kLdsPerCu;
+        int best = 0, best_rows = 0;
+        for (int r = 1; r * G <= 256; ++r) {
+            if ((r * G) % 64 != 0 || r * rowb > cap) continue;
+            const int by_lds = (kLdsPerCu / (r * rowb)) * r;
+            const int by_waves = kWavesPerCu * 64 / G;
+            const int resident = by_lds < by_waves ? by_lds : by_waves;
+            if (resident > best_rows || (pass == 0 && resident == best_rows)) {
+                best = r;
+                best_rows = resident;
+            }
+        }
+        if (best > 0) return best;
+    }
+    return 64 / G;
+}
+
 template <int G, int CAP>
 static void launch_rankw(hipStream_t st, const NumArgs &a, const int32_t *rows, int nrows)
 {
     constexpr int ROWB = kRankWords * 6 + CAP * 4 + (G < 64 ? G * 16 : 0);
-    constexpr int RMAX = (256 / G) < (49152 / ROWB) ? (256 / G) : (49152 / ROWB);
-    constexpr int R = (G == 32 && RMAX > 1) ? (RMAX & ~1) : (RMAX > 0 ? RMAX : 1);
+    constexpr int R = rows_for_lds<G>(ROWB);
     static_assert((R * G) % 64 == 0, "block must be whole waves");
     k_num_rankw<G, CAP, R><<<(nrows + R - 1) / R, R * G, 0, st>>>(rows, nrows, a);
 }
@@ -293,8 +321,8 @@ static void launch_rankw(hipStream_t st, const NumArgs &a, const int32_t *rows, 
 template <int G, int H>
 static void launch_hash(hipStream_t st, const NumArgs &a, const int32_t *rows, int nrows)
 {
-    constexpr int R = (256 / G) < (8192 / H) ? (256 / G) : (8192 / H);
-    static_assert(R * G >= 64, "block must hold at least one full wave");
+    constexpr int R = rows_for_lds<G>(8 * H + (G < 64 ? G * 16 : 0));
+    static_assert((R * G) % 64 == 0, "block must be whole waves");
     const int grid = (nrows + R - 1) / R;
     k_num_hash<G, H, R><<<grid, R * G, 0, st>>>(rows, nrows, a);
 }
@@ -302,8 +330,8 @@ static void launch_hash(hipStream_t st, const NumArgs &a, const int32_t *rows, i
 template <int G, int W>
 static void launch_dense(hipStream_t st, const NumArgs &a, const int32_t *rows, int nrows)
 {
-    constexpr int R = (256 / G) < (16384 / W) ? (256 / G) : (16384 / W);
-    static_assert(R * G >= 64, "block must hold at least one full wave");
+    constexpr int R = rows_for_lds<G>(4 * W + (G < 64 ? G * 16 : 0));
+    static_assert((R * G) % 64 == 0, "block must be whole waves");
     const int grid = (nrows + R - 1) / R;
     k_num_dense<G, W, R><<<grid, R * G, 0, st>>>(rows, nrows, a);
 }
