@@ -43,8 +43,8 @@ RANK_MAX_NNZ = 4096
 
 
 def training_products():
-    """(name, A, B) host products: the benchmark families at other seeds
-    and sizes."""
+    """(name, A, B) host products: the benchmark families and rv-sparse's
+    generated inputs, at other seeds and sizes."""
     out = []
     for scale, ef, seed in ((15, 16, 101), (16, 8, 102), (14, 32, 103)):
         A = gen_rmat(scale, ef << scale, 0.57, 0.19, 0.19, seed)
@@ -60,6 +60,12 @@ def training_products():
         out.append((f"rows{n}_{lo}_{hi}", A, B))
     A = gen_stencil27(32, 131, normal=True)
     out.append(("stencil32", A, A))
+    # rv-sparse's own generated inputs (log-normal degrees, near-dense rows),
+    # at other seeds than its bench sweep
+    from . import gen_bench_pair
+    for r, d, seed in ((4096, 0.002, 141), (2048, 0.005, 143)):
+        A, B = gen_bench_pair(r, r, d, seed)
+        out.append((f"genmat{r}", A, B))
     return out
 
 
