@@ -40,6 +40,7 @@ HASH_MAX_NNZ = 2730
 DENSE_MAX_COLS = 16384
 RANK_MAX_COLS = 16384
 RANK_MAX_NNZ = 4096
+RANK_WORDS = RANK_MAX_COLS // 32
 
 
 def training_products():
@@ -63,7 +64,7 @@ def training_products():
     # rv-sparse's own generated inputs (log-normal degrees, near-dense rows),
     # at other seeds than its bench sweep
     from . import gen_bench_pair
-    for r, d, seed in ((4096, 0.002, 141), (2048, 0.005, 143)):
+    for r, d, seed in ((4096, 0.002, 141), (2048, 0.005, 143), (1024, 0.01, 145), (512, 0.02, 147)):
         A, B = gen_bench_pair(r, r, d, seed)
         out.append((f"genmat{r}", A, B))
     return out
@@ -113,7 +114,8 @@ def _next_pow2(v):
 def chooser_bins(nz, span):
     """(m, 8) numeric bin of each chooser output per row, -1 if infeasible;
     the host image of chooser_bin in rvsp_mlp.hpp (outputs 0-3 hash table,
-    4-7 window: a rank window where it fits with 32/64 lanes, else dense)."""
+    4-7 window: a rank window where it fits with 32/64 lanes and its LDS
+    footprint is no larger than a dense window's, else dense)."""
     nz = np.asarray(nz, np.int64)
     span = np.asarray(span, np.int64)
     H = np.maximum(64, _next_pow2((3 * nz + 1) // 2))
@@ -134,7 +136,9 @@ def chooser_bins(nz, span):
             C = np.maximum(512, _next_pow2(nz))
             lc = np.log2(C).astype(np.int64)
             rank_ok = (span <= RANK_MAX_COLS) & (nz <= RANK_MAX_NNZ) & (G >= 32)
-            out[:, o] = np.where(rank_ok, NB_RANK0 + 2 * (lc - 9) + (1 if G == 64 else 0), dense)
+            # the smaller LDS footprint, as the heuristic decides it
+            rank_first = rank_ok & (~ok | (RANK_WORDS * 6 + 4 * C <= 4 * W))
+            out[:, o] = np.where(rank_first, NB_RANK0 + 2 * (lc - 9) + (1 if G == 64 else 0), dense)
     return out
 
 

@@ -68,8 +68,9 @@ __device__ __forceinline__ void row_features(int64_t ops, int alen, int64_t span
 
 // Numeric bin of chooser output o for a row, or -1 when not feasible
 // (only the instantiated (H, G), (W, G) and (CAP, G) triples are). Outputs
-// 0-3: hash table; 4-7: window (rank window when the row fits one and G is
-// 32 or 64, else dense window).
+// 0-3: hash table; 4-7: window (rank window when the row fits one, G is 32
+// or 64 and its LDS footprint is no larger than a dense window's, else
+// dense window).
 __device__ __forceinline__ int chooser_bin(int o, int nz, int64_t span)
 {
     const int lg = 3 + (o & 3);
@@ -79,14 +80,16 @@ __device__ __forceinline__ int chooser_bin(int o, int nz, int64_t span)
         if ((1 << lg) < (H >= 4096 ? 32 : H >= 2048 ? 16 : 8)) return -1;
         return NB_HASH0 + 4 * (ilog2(H) - 6) + (lg - 3);
     }
-    // window kind: a rank window where it fits (32 or 64 lanes), else dense
-    if (span <= kRankMaxCols && nz <= kRankMaxNnz && lg >= 5) {
-        const int C = max(512, next_pow2(nz));
+    // window kind by LDS footprint, as the heuristic decides it: a rank
+    // window (32 or 64 lanes) unless a dense window over the span is smaller
+    const bool rank_ok = span <= kRankMaxCols && nz <= kRankMaxNnz && lg >= 5;
+    const int C = max(512, next_pow2(nz));
+    const int W = span <= kDenseMaxCols ? max(512, next_pow2((int)span)) : 0;
+    const bool dense_ok =
+        W > 0 && (1 << lg) >= (W >= 16384 ? 64 : W >= 8192 ? 32 : W >= 4096 ? 16 : 8);
+    if (rank_ok && (!dense_ok || kRankWords * 6 + 4 * C <= 4 * W))
         return NB_RANK0 + 2 * (ilog2(C) - 9) + (lg == 6 ? 1 : 0);
-    }
-    if (span > kDenseMaxCols) return -1;
-    const int W = max(512, next_pow2((int)span));
-    if ((1 << lg) < (W >= 16384 ? 64 : W >= 8192 ? 32 : W >= 4096 ? 16 : 8)) return -1;
+    if (!dense_ok) return -1;
     return NB_DENSE0 + 4 * (ilog2(W) - 9) + (lg - 3);
 }
 

@@ -107,19 +107,23 @@ def brute_chooser_bin(o, nz, span):
         if G < (32 if H >= 4096 else 16 if H >= 2048 else 8):
             return -1
         return 1 + 4 * (int(np.log2(H)) - 6) + (o & 3)
-    if span <= 16384 and nz <= 4096 and G >= 32:
-        C = 512
-        while C < nz:
-            C *= 2
-        return 53 + 2 * (int(np.log2(C)) - 9) + (1 if G == 64 else 0)
-    if span > 16384:
-        return -1
-    W = 512
-    while W < span:
-        W *= 2
-    if G < (64 if W >= 16384 else 32 if W >= 8192 else 16 if W >= 4096 else 8):
-        return -1
-    return 29 + 4 * (int(np.log2(W)) - 9) + (o & 3)
+    C = 512
+    while C < nz:
+        C *= 2
+    rank = 53 + 2 * (int(np.log2(C)) - 9) + (1 if G == 64 else 0)
+    rank_ok = span <= 16384 and nz <= 4096 and G >= 32
+    dense = -1
+    if span <= 16384:
+        W = 512
+        while W < span:
+            W *= 2
+        if G >= (64 if W >= 16384 else 32 if W >= 8192 else 16 if W >= 4096 else 8):
+            dense = 29 + 4 * (int(np.log2(W)) - 9) + (o & 3)
+    # the window with the smaller LDS footprint: rank 512 words x 6 bytes + 4 per
+    # output slot, dense 4 per column (the rank window on a tie)
+    if rank_ok and (dense < 0 or 512 * 6 + 4 * C <= 4 * W):
+        return rank
+    return dense
 
 
 def test_chooser_candidates_match_brute_force():
