@@ -143,20 +143,25 @@ void rvsp_spgemm_batch_destroy(rvsp_spgemm_batch_t batch);
 /* ---- learned row selectors ------------------------------------------------ */
 
 /*
- * RVSP_SPGEMM_ALGO_HIP_LEARNED uses two small MLPs (features -> 16 -> 16 ->
- * outputs, ReLU) in place of fixed binning rules:
+ * RVSP_SPGEMM_ALGO_HIP_LEARNED, and RVSP_SPGEMM_ALGO_DEFAULT, use two small
+ * MLPs (features -> 16 -> 16 -> outputs, ReLU) in place of fixed binning
+ * rules where their parameters are loaded; RVSP_SPGEMM_ALGO_HIP always uses
+ * the rules:
  *   SIZER   4 features -> 1: predicted log2(nnz(C_i) / min(ops_i, span_i)),
  *           sizes each row's symbolic LDS hash table (times the margin);
  *           rows whose exact count overflows are recounted in their
  *           heuristic bin, so results never depend on the prediction;
  *   CHOOSER 5 features -> 8: predicted log2(ns per row) of each numeric
- *           accumulator (LDS hash table or dense window, 8..64 lanes);
- *           the smallest feasible prediction wins (the margin is unused).
+ *           accumulator (LDS hash table, or the rank or dense window the
+ *           rules would use, 8..64 lanes); the smallest feasible prediction
+ *           wins unless it beats the rules' own pick by less than the
+ *           margin.
  * Feature vectors are 5 floats per row: log2(1+alen)/16, log2(1+ops)/32,
  * log2(1+span)/32, log2(1+ops/alen)/16, log2(1+nnz)/32 (the sizer reads
  * the first four). Parameters are stored layer by layer: W1[16][nf], b1,
  * W2[16][16], b2, W3[no][16], b3.
- * Without loaded parameters the learned algo behaves as RVSP_SPGEMM_ALGO_HIP.
+ * Without loaded parameters both behave as RVSP_SPGEMM_ALGO_HIP. Results do
+ * not depend on the choice: they are the same bit for bit.
  * Parameters load on first use from the file named by RVSP_SELECTOR_SIZER /
  * RVSP_SELECTOR_CHOOSER, else from selector/{sizer,chooser}.txt beside the
  * library's directory (text: the margin, then the parameters).

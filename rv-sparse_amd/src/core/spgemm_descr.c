@@ -218,9 +218,11 @@ rvsp_status_t rvsp_spgemm_work_estimation(rvsp_spgemm_descr_t descr, const rvsp_
     }
 
     int64_t c_nnz = 0, products = 0;
-    const rvsp_select_t sel = descr->algo == RVSP_SPGEMM_ALGO_HIP_LEARNED
-                                  ? RVSP_SELECT_LEARNED
-                                  : RVSP_SELECT_HEURISTIC;
+    /* RVSP_SPGEMM_ALGO_HIP forces the heuristic bins; the default and the
+       learned algo take the learned selectors when their weights are loaded
+       (the heuristic otherwise). The result is the same bit for bit. */
+    const rvsp_select_t sel = descr->algo == RVSP_SPGEMM_ALGO_HIP ? RVSP_SELECT_HEURISTIC
+                                                                  : RVSP_SELECT_LEARNED;
     s = rvsp_hip_plan_analyse(descr->plan, A->rows, A->cols, B->cols, A->nnz, arp, aci, B->nnz,
                               brp, bci, sel, &c_nnz, &products);
     if (s != RVSP_SUCCESS)
