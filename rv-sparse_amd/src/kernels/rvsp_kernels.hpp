@@ -115,6 +115,11 @@ void launch_interleave(hipStream_t st, int32_t nnz, const int32_t *bci, const fl
 // One hash or dense numeric bin over the given rows (false: not such a bin).
 bool launch_num_bin(hipStream_t st, const NumArgs &a, int bin, const int32_t *rows, int nrows);
 void launch_sym_big(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g);
+// Reorders the big-row list largest first (buckets of a quarter octave of
+// products); work holds kBigOrderWork ints, tmp nrows ints.
+constexpr int kBigOrderWork = 512;
+hipError_t launch_big_order(hipStream_t st, int32_t *rows, int nrows, const int64_t *ops,
+                            int32_t *work, int32_t *tmp);
 // Big-row structure fill over every planned segment (g.nsegs).
 void launch_big_fill(hipStream_t st, const BigArgs &g, const int32_t *crp, int32_t *cci);
 void launch_big_plan(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,

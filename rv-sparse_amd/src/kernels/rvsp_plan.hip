@@ -419,6 +419,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     const size_t o_tsum = off; off = align256(off + (size_t)ntiles * 8);
     const size_t o_ctr = off; off = align256(off + sizeof(AnalysisCounters));
     const size_t o_binoff = off; off = align256(off + 128 * 4);
+    const size_t o_border = off; off = align256(off + kBigOrderWork * 4);
     const size_t o_err = off; off = align256(off + 4);
     rvsp_status_t s = dalloc(&p->rowblock, off, st);
     if (s != RVSP_SUCCESS) return s;
@@ -437,6 +438,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->tsum = (int64_t *)(base + o_tsum);
     p->ctr = (AnalysisCounters *)(base + o_ctr);
     p->binoff = (int32_t *)(base + o_binoff);
+    int32_t *border = (int32_t *)(base + o_border);
     p->err = (int *)(base + o_err);
 
     PhaseScope timer(st, PH_ANALYSE);
@@ -478,6 +480,9 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
         HIP_TRY(hipMemsetAsync(p->binoff + 64, 0, 64 * 4, st));
         launch_bin_scatter(st, m, p->sbin, p->binoff, p->binoff + 64, p->sperm, SB_COUNT);
         HIP_TRY(hipGetLastError());
+        // big rows largest first (nperm is free until the numeric bins)
+        HIP_TRY(launch_big_order(st, p->sperm + p->sym_off[SB_BIG], p->sym_count[SB_BIG], p->ops,
+                                 border, p->nperm));
         return RVSP_SUCCESS;
     };
     s = scatter_sym();
