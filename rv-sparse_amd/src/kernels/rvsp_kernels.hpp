@@ -85,6 +85,34 @@ using NumArgs = NumArgsT<float, float, int2>;      // rv-sparse's FP32 contract
 using NumArgsD = NumArgsT<double, double, PairD>;  // legacy FP64
 using NumArgsI = NumArgsT<int8_t, int32_t, int2>;  // legacy INT8 -> INT32
 
+// Independent bin launches spread over the caller's stream and up to three
+// side streams of the device (default priority): forked from the caller's
+// stream on construction, joined back to it on destruction. One scope per
+// device at a time (a lock is held from fork to join, so host threads that
+// share the side streams cannot cross their events); no fan-out while the
+// caller's stream is being captured into a graph. RVSP_NUM_STREAMS (1-4,
+// default 4) sets the streams in use.
+class StreamFan {
+public:
+    StreamFan(hipStream_t st, bool want);
+    ~StreamFan();
+    StreamFan(const StreamFan &) = delete;
+    StreamFan &operator=(const StreamFan &) = delete;
+    // stream for the i-th independent launch (the side streams in turn)
+    hipStream_t pick(int i) const { return n_ <= 1 ? main_ : side_[i % (n_ - 1)]; }
+
+private:
+    hipStream_t main_;
+    int n_ = 1, dev_ = -1;
+    hipStream_t side_[3] = {};
+    hipEvent_t join_[3] = {};
+};
+// Passes with fewer products run on one stream: the fork and join (about
+// 50 us) would cost more than the launches' overlap saves.
+constexpr int64_t kFanMinProducts = int64_t(1) << 21;
+// The structure fill writes nnz(C) columns; below this it runs on one stream.
+constexpr int64_t kFanMinFill = int64_t(1) << 24;
+
 void launch_row_stats(hipStream_t st, int32_t m, const int32_t *arp, const int32_t *aci,
                       const int32_t *brp, const int32_t *bci, int64_t *ops, int32_t *lo,
                       int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr);
@@ -93,10 +121,10 @@ void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int
 // learned: hash bins use the overflow-checking count (a row whose count
 // exceeds H/2 reports H + 1 instead of raising an error).
 void launch_sym_count(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
-                      const int32_t *off, bool learned);
+                      const int32_t *off, bool learned, bool fan = false);
 // skip_big: the big rows' columns are left to a numeric pass with cols_out set.
 void launch_sym_fill(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
-                     const int32_t *off, bool skip_big = false);
+                     const int32_t *off, bool skip_big = false, bool fan = false);
 void launch_scan(hipStream_t st, int32_t m, const int32_t *cnt, int64_t *tsum, int64_t *total,
                  int32_t *crp);
 int scan_tiles(int32_t m);
@@ -108,7 +136,7 @@ void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_
                      const int32_t *hi, const float *chooser, float lg_margin, uint8_t *nbin,
                      AnalysisCounters *ctr);
 void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
-                    const int32_t *off, const BigArgs &big, int nsegs);
+                    const int32_t *off, const BigArgs &big, int nsegs, bool fan = false);
 // Writes B's (column, value) pairs for a numeric pass: bcv[i] = (bci[i], bval[i]).
 void launch_interleave(hipStream_t st, int32_t nnz, const int32_t *bci, const float *bval,
                        int2 *bcv);

@@ -21,6 +21,10 @@
 //   rank    span <= 16384 and nnz(C_i) <= 4096: a bitmap of C_i's columns
 //           with a popcount prefix maps a column to its output position
 //   big     rows the symbolic pass kept a bitmap for (rvsp_bigrow.hip)
+#include <stdlib.h>
+
+#include <mutex>
+
 #include "rvsp_device.hpp"
 #include "rvsp_kernels.hpp"
 
@@ -423,12 +427,82 @@ void launch_interleave(hipStream_t st, int32_t nnz, const int32_t *bci, const fl
     if (nnz > 0) k_interleave<<<(int)(((int64_t)nnz + 255) / 256), 256, 0, st>>>(nnz, bci, bval, bcv);
 }
 
-void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
-                    const int32_t *off, const BigArgs &big, int nsegs)
+namespace {
+constexpr int kFanDevices = 16;
+struct FanPool {
+    std::mutex mu;
+    bool tried = false, ok = false;
+    hipStream_t side[3] = {};
+    hipEvent_t fork = nullptr, join[3] = {};
+};
+FanPool g_fan[kFanDevices];
+
+int fan_streams()
 {
+    static const int n = [] {
+        const char *e = getenv("RVSP_NUM_STREAMS");
+        const int v = e ? atoi(e) : 4;
+        return v < 1 ? 1 : v > 4 ? 4 : v;
+    }();
+    return n;
+}
+}  // namespace
+
+StreamFan::StreamFan(hipStream_t st, bool want) : main_(st)
+{
+    const int nwant = want ? fan_streams() : 1;
+    if (nwant <= 1) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kFanDevices) return;
+    FanPool &p = g_fan[dev];
+    p.mu.lock();
+    if (!p.tried) {
+        p.tried = true;
+        bool good = hipEventCreateWithFlags(&p.fork, hipEventDisableTiming) == hipSuccess;
+        for (int i = 0; i < 3 && good; ++i)
+            good = hipStreamCreateWithFlags(&p.side[i], hipStreamNonBlocking) == hipSuccess &&
+                   hipEventCreateWithFlags(&p.join[i], hipEventDisableTiming) == hipSuccess;
+        p.ok = good;
+    }
+    int k = 0;
+    if (p.ok && hipEventRecord(p.fork, st) == hipSuccess) {
+        for (int i = 0; i < nwant - 1; ++i) {
+            if (hipStreamWaitEvent(p.side[i], p.fork, 0) != hipSuccess) break;
+            side_[k] = p.side[i];
+            join_[k] = p.join[i];
+            ++k;
+        }
+    }
+    if (k == 0) {
+        p.mu.unlock();
+        return;
+    }
+    n_ = 1 + k;
+    dev_ = dev;
+}
+
+StreamFan::~StreamFan()
+{
+    if (dev_ < 0) return;
+    for (int i = 0; i < n_ - 1; ++i) {
+        // a join that cannot be queued is made on the host instead
+        if (hipEventRecord(join_[i], side_[i]) != hipSuccess ||
+            hipStreamWaitEvent(main_, join_[i], 0) != hipSuccess)
+            (void)hipStreamSynchronize(side_[i]);
+    }
+    g_fan[dev_].mu.unlock();
+}
+
+void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
+                    const int32_t *off, const BigArgs &big, int nsegs, bool fan)
+{
+    const StreamFan f(st, fan);
+    launch_num_big(st, nsegs, a, big);  // the longest work first, on the caller's stream
+    int i = 0;
     for (int b = NB_HASH0; b < NB_WIN; ++b)
-        if (count[b] > 0) (void)launch_num_bin(st, a, b, perm + off[b], count[b]);
-    launch_num_big(st, nsegs, a, big);
+        if (count[b] > 0) (void)launch_num_bin(f.pick(i++), a, b, perm + off[b], count[b]);
 }
 
 } // namespace rvsp

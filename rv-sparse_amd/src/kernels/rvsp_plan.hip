@@ -553,7 +553,8 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     // rows without products get no kernel, so their count must start at 0
     HIP_TRY(hipMemsetAsync(p->cnt, 0, M * 4, st));
     const bool sized = sizer && p->hctr.moved > 0;
-    launch_sym_count(st, sa, p->sperm, p->sym_count, p->sym_off, sized);
+    launch_sym_count(st, sa, p->sperm, p->sym_count, p->sym_off, sized,
+                     p->products >= kFanMinProducts);
     HIP_TRY(hipGetLastError());
     if (sized) {
         // rows whose learned table overflowed go back to their heuristic
@@ -583,7 +584,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
             // nperm is free until the numeric bins are scattered
             launch_bin_scatter(st, m, p->fbin, p->binoff, p->binoff + 64, p->nperm, SB_COUNT);
             HIP_TRY(hipGetLastError());
-            launch_sym_count(st, sa, p->nperm, fcount, foff, false);
+            launch_sym_count(st, sa, p->nperm, fcount, foff, false, p->products >= kFanMinProducts);
             HIP_TRY(hipGetLastError());
             s = scatter_sym();
             if (s != RVSP_SUCCESS) return s;
@@ -651,7 +652,8 @@ rvsp_status_t rvsp_hip_plan_fill(rvsp_hip_plan_t *p, int32_t *cci)
     SymArgs sa{p->arp, p->aci, p->brp, p->bci, p->lo,  p->hi,  p->cnt,
                p->crp, cci,    p->big, p->err, p->ops, p->bms_spill_ok ? p->bms_spill : nullptr};
     PhaseScope timer(p->stream, PH_FILL);
-    launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off);
+    launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off, false,
+                    p->nnzc >= kFanMinFill);
     HIP_TRY(hipGetLastError());
     return RVSP_SUCCESS;
 }
@@ -677,7 +679,8 @@ static void queue_numeric(rvsp_hip_plan *p, const float *aval, const float *bval
         launch_numeric_ranked(p->stream, a, p->nperm, p->num_count, p->num_off, big, p->nseg,
                               p->prow, p->pmap);
     else
-        launch_numeric(p->stream, a, p->nperm, p->num_count, p->num_off, big, p->nseg);
+        launch_numeric(p->stream, a, p->nperm, p->num_count, p->num_off, big, p->nseg,
+                       p->products >= kFanMinProducts);
 }
 
 rvsp_status_t rvsp_hip_plan_numeric(rvsp_hip_plan_t *p, const float *aval, const float *bval,
@@ -700,7 +703,8 @@ rvsp_status_t rvsp_hip_plan_fill_numeric(rvsp_hip_plan_t *p, const float *aval,
         SymArgs sa{p->arp, p->aci, p->brp, p->bci, p->lo,  p->hi,  p->cnt,
                    p->crp, cci,    p->big, p->err, p->ops, p->bms_spill_ok ? p->bms_spill : nullptr};
         PhaseScope timer(p->stream, PH_FILL);
-        launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off, /*skip_big=*/true);
+        launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off, /*skip_big=*/true,
+                        p->nnzc >= kFanMinFill);
         HIP_TRY(hipGetLastError());
     }
     PhaseScope timer(p->stream, PH_NUMERIC);

@@ -660,13 +660,20 @@ static void launch_sym_learned(hipStream_t st, const SymArgs &a, int b, const in
 }
 
 template <bool FILL>
-static void launch_sym_bins(hipStream_t st, const SymArgs &a, const int32_t *perm,
+static void launch_sym_bins(hipStream_t main, const SymArgs &a, const int32_t *perm,
                             const int32_t *count, const int32_t *off, bool learned = false,
-                            bool skip_big = false)
+                            bool skip_big = false, bool fan = false)
 {
-    for (int b = SB_H64; b < SB_COUNT; ++b) {
+    // bins touch disjoint rows: the big rows (the longest work) first on the
+    // caller's stream, the others over the side streams
+    const StreamFan f(main, fan);
+    static_assert(SB_BIG == SB_COUNT - 1 && SB_BMS == SB_BIG - 1, "big rows are the last bin");
+    int i = 0;
+    for (int bi = 0; bi < SB_COUNT - SB_H64; ++bi) {
+        const int b = bi == 0 ? SB_BIG : SB_H64 + bi - 1;  // SB_BIG, then SB_H64..SB_BMS
         const int nr = count[b];
         if (nr == 0 || (skip_big && b == SB_BIG)) continue;
+        const hipStream_t st = b == SB_BIG ? main : f.pick(i++);
         const int32_t *rows = perm + off[b];
         if (!FILL && learned && b >= SB_H64 && b <= SB_H4096) {
             launch_sym_learned(st, a, b, rows, nr);
@@ -714,15 +721,15 @@ void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int
 }
 
 void launch_sym_count(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
-                      const int32_t *off, bool learned)
+                      const int32_t *off, bool learned, bool fan)
 {
-    launch_sym_bins<false>(st, a, perm, count, off, learned);
+    launch_sym_bins<false>(st, a, perm, count, off, learned, false, fan);
 }
 
 void launch_sym_fill(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
-                     const int32_t *off, bool skip_big)
+                     const int32_t *off, bool skip_big, bool fan)
 {
-    launch_sym_bins<true>(st, a, perm, count, off, false, skip_big);
+    launch_sym_bins<true>(st, a, perm, count, off, false, skip_big, fan);
 }
 
 void launch_scan(hipStream_t st, int32_t m, const int32_t *cnt, int64_t *tsum, int64_t *total,
