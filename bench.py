@@ -475,7 +475,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "numeric pass (row-binned LDS accumulator kernels, one stream)",
+            "kernel": "numeric pass (row-binned LDS accumulator kernels; HIP events on the caller's stream, which joins the bins' side streams)",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",

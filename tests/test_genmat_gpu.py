@@ -52,6 +52,11 @@ def test_large_generated_pairs_match_oracle(R, Cc, D, S):
     A, B = rvsp.gen_bench_pair(R, Cc, D, S)
     dA, dB = rvsp.DeviceCsr(A), rvsp.DeviceCsr(B)
     want = pyoracle.spgemm(A, B)
+    if (R, Cc) == (16384, 16384):
+        # above both thresholds for running row bins on concurrent streams
+        # (2^21 products, 2^24 outputs): count, fill and numeric all fan out
+        products = int(np.diff(B.row_ptr)[A.col_idx].astype(np.int64).sum())
+        assert products >= 1 << 21 and int(want[0][-1]) >= 1 << 24
     for algo in (rvsp.ALGO_HIP, rvsp.ALGO_HIP_LEARNED):
         assert_bitexact(rvsp.spgemm_device(dA, dB, algo=algo), want, f"oracle algo {algo}")
 
