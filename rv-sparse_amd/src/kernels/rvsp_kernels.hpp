@@ -184,7 +184,8 @@ void launch_pm_build(hipStream_t st, int32_t m, const uint8_t *nbin, const int64
 int pm_scan_tiles(int32_t m);
 void launch_numeric_ranked(hipStream_t st, const NumArgs &a, const int32_t *perm,
                            const int32_t *count, const int32_t *off, const BigArgs &big,
-                           int nsegs, const int64_t *prow, const uint16_t *pmap);
+                           int nsegs, const int64_t *prow, const uint16_t *pmap,
+                           bool fan = false);
 
 // learned selectors (rvsp_selector.hip)
 const float *selector_device_params(int which, float *margin);

@@ -677,7 +677,7 @@ static void queue_numeric(rvsp_hip_plan *p, const float *aval, const float *bval
     const NumArgs a = num_args(p, aval, bval, cci, cval);
     if (p->pm_ready)
         launch_numeric_ranked(p->stream, a, p->nperm, p->num_count, p->num_off, big, p->nseg,
-                              p->prow, p->pmap);
+                              p->prow, p->pmap, p->products >= kFanMinProducts);
     else
         launch_numeric(p->stream, a, p->nperm, p->num_count, p->num_off, big, p->nseg,
                        p->products >= kFanMinProducts);
