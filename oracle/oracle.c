@@ -151,6 +151,60 @@ int orc_symbolic_count(int32_t m, int32_t n, const int32_t *a_rp,
     return 0;
 }
 
+/* Row-parallel orc_symbolic_count (OpenMP, per-thread mark arrays): the
+ * same per-row counts and op counts, and the same exclusive scan and int32
+ * overflow rule, evaluated over `threads` threads. Used to check products
+ * whose size makes the sequential count too slow for a test. */
+int orc_symbolic_count_par(int32_t m, int32_t n, const int32_t *a_rp,
+                           const int32_t *a_ci, const int32_t *b_rp,
+                           const int32_t *b_ci, int32_t *c_rp, int64_t *ops,
+                           int64_t *total_out, int threads)
+{
+    size_t nn = (size_t)(n > 0 ? n : 1);
+    int fail = 0;
+    if (threads < 1)
+        threads = 1;
+#pragma omp parallel num_threads(threads)
+    {
+        unsigned char *mark = (unsigned char *)calloc(nn, 1);
+        int32_t *touched = (int32_t *)malloc(nn * sizeof(int32_t));
+        if (!mark || !touched)
+        {
+#pragma omp atomic write
+            fail = 1;
+        }
+        else
+        {
+#pragma omp for schedule(dynamic, 64)
+            for (int32_t r = 0; r < m; r++)
+            {
+                int64_t o;
+                c_rp[r] = (int32_t)count_row(r, a_rp, a_ci, b_rp, b_ci, mark,
+                                             touched, &o);
+                if (ops)
+                    ops[r] = o;
+            }
+        }
+        free(mark);
+        free(touched);
+    }
+    if (fail)
+        return -6;
+    int64_t run = 0;
+    for (int32_t r = 0; r < m; r++)
+    {
+        int32_t c = c_rp[r];
+        c_rp[r] = (int32_t)run;
+        run += c;
+        if (run > INT32_MAX)
+            return -6;
+    }
+    c_rp[m] = (int32_t)run;
+    if (total_out)
+        *total_out = run;
+    return 0;
+}
+
 /* Ascending sort of distinct column ids: insertion sort for short lists,
  * byte-wise LSD radix sort otherwise (rvsp_sort.h:16-124 uses the same two
  * regimes, with the switch at 40 entries and as many passes as the largest
@@ -279,6 +333,28 @@ void orc_numeric_f32(int32_t m, int32_t n, const int32_t *a_rp,
     free(acc);
 }
 
+/* One output row of the FP64 yardstick: products and sums in double, the
+ * same A-entry order and drain as numeric_row. */
+static void numeric_row_f64(int32_t r, const int32_t *a_rp,
+                            const int32_t *a_ci, const float *a_val,
+                            const int32_t *b_rp, const int32_t *b_ci,
+                            const float *b_val, const int32_t *c_rp,
+                            const int32_t *c_ci, double *c_val, double *acc)
+{
+    for (int32_t ap = a_rp[r]; ap < a_rp[r + 1]; ap++)
+    {
+        int32_t k = a_ci[ap];
+        double a = (double)a_val[ap];
+        for (int32_t bp = b_rp[k]; bp < b_rp[k + 1]; bp++)
+            acc[b_ci[bp]] += a * (double)b_val[bp];
+    }
+    for (int32_t q = c_rp[r]; q < c_rp[r + 1]; q++)
+    {
+        c_val[q] = acc[c_ci[q]];
+        acc[c_ci[q]] = 0.0;
+    }
+}
+
 void orc_numeric_f64(int32_t m, int32_t n, const int32_t *a_rp,
                      const int32_t *a_ci, const float *a_val,
                      const int32_t *b_rp, const int32_t *b_ci,
@@ -289,21 +365,57 @@ void orc_numeric_f64(int32_t m, int32_t n, const int32_t *a_rp,
     if (!acc)
         return;
     for (int32_t r = 0; r < m; r++)
-    {
-        for (int32_t ap = a_rp[r]; ap < a_rp[r + 1]; ap++)
-        {
-            int32_t k = a_ci[ap];
-            double a = (double)a_val[ap];
-            for (int32_t bp = b_rp[k]; bp < b_rp[k + 1]; bp++)
-                acc[b_ci[bp]] += a * (double)b_val[bp];
-        }
-        for (int32_t q = c_rp[r]; q < c_rp[r + 1]; q++)
-        {
-            c_val[q] = acc[c_ci[q]];
-            acc[c_ci[q]] = 0.0;
-        }
-    }
+        numeric_row_f64(r, a_rp, a_ci, a_val, b_rp, b_ci, b_val, c_rp, c_ci,
+                        c_val, acc);
     free(acc);
+}
+
+/* Row-parallel fill + FP32 numeric (+ FP64 numeric when c_val64 != NULL)
+ * on a row_ptr from the symbolic count: every row is computed by the same
+ * fill_row / numeric_row code as orc_symbolic_fill / orc_numeric_f32, so the
+ * outputs are identical; rows are shared out over `threads` OpenMP threads.
+ * Returns 0, or -6 when a thread's scratch cannot be allocated. */
+int orc_fill_numeric_par(int32_t m, int32_t n, const int32_t *a_rp,
+                         const int32_t *a_ci, const float *a_val,
+                         const int32_t *b_rp, const int32_t *b_ci,
+                         const float *b_val, const int32_t *c_rp,
+                         int32_t *c_ci, float *c_val, double *c_val64,
+                         int threads)
+{
+    size_t nn = (size_t)(n > 0 ? n : 1);
+    int fail = 0;
+    if (threads < 1)
+        threads = 1;
+#pragma omp parallel num_threads(threads)
+    {
+        unsigned char *mark = (unsigned char *)calloc(nn, 1);
+        int32_t *tmp = (int32_t *)malloc(nn * sizeof(int32_t));
+        float *acc = (float *)calloc(nn, sizeof(float));
+        double *acc64 = c_val64 ? (double *)calloc(nn, sizeof(double)) : NULL;
+        if (!mark || !tmp || !acc || (c_val64 && !acc64))
+        {
+#pragma omp atomic write
+            fail = 1;
+        }
+        else
+        {
+#pragma omp for schedule(dynamic, 16)
+            for (int32_t r = 0; r < m; r++)
+            {
+                fill_row(r, n, a_rp, a_ci, b_rp, b_ci, c_rp, c_ci, mark, tmp);
+                numeric_row(r, a_rp, a_ci, a_val, b_rp, b_ci, b_val, c_rp,
+                            c_ci, c_val, acc);
+                if (c_val64)
+                    numeric_row_f64(r, a_rp, a_ci, a_val, b_rp, b_ci, b_val,
+                                    c_rp, c_ci, c_val64, acc64);
+            }
+        }
+        free(mark);
+        free(tmp);
+        free(acc);
+        free(acc64);
+    }
+    return fail ? -6 : 0;
 }
 
 int orc_spgemm_f32(int32_t m, int32_t k, int32_t n, const int32_t *a_rp,

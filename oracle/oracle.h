@@ -57,6 +57,23 @@ void orc_numeric_f64(int32_t m, int32_t n, const int32_t *a_rp,
                      const float *b_val, const int32_t *c_rp,
                      const int32_t *c_ci, double *c_val);
 
+/* Row-parallel orc_symbolic_count over `threads` OpenMP threads: same
+ * outputs and return codes. */
+int orc_symbolic_count_par(int32_t m, int32_t n, const int32_t *a_rp,
+                           const int32_t *a_ci, const int32_t *b_rp,
+                           const int32_t *b_ci, int32_t *c_rp, int64_t *ops,
+                           int64_t *total_out, int threads);
+
+/* Row-parallel orc_symbolic_fill + orc_numeric_f32 (+ orc_numeric_f64 into
+ * c_val64 when it is not NULL) on a counted c_rp: same outputs, rows shared
+ * over `threads` OpenMP threads. Returns 0, or -6 when scratch is short. */
+int orc_fill_numeric_par(int32_t m, int32_t n, const int32_t *a_rp,
+                         const int32_t *a_ci, const float *a_val,
+                         const int32_t *b_rp, const int32_t *b_ci,
+                         const float *b_val, const int32_t *c_rp,
+                         int32_t *c_ci, float *c_val, double *c_val64,
+                         int threads);
+
 /* Full one-shot like rvsp_spgemm_scalar_f32 (gustavson_core_f32.inc:66-194):
  * count, malloc C, fill + sort, numeric. Outputs are malloc'd; release with
  * orc_free. Returns 0, -2 on bad sizes, -6 on allocation failure/overflow. */

@@ -37,6 +37,10 @@ def oracle():
         o.orc_numeric_f32.argtypes = [I32, I32, V, V, V, V, V, V, V, V, V]
         o.orc_numeric_f64.restype = None
         o.orc_numeric_f64.argtypes = [I32, I32, V, V, V, V, V, V, V, V, V]
+        o.orc_symbolic_count_par.restype = C.c_int
+        o.orc_symbolic_count_par.argtypes = [I32, I32, V, V, V, V, V, V, C.POINTER(I64), C.c_int]
+        o.orc_fill_numeric_par.restype = C.c_int
+        o.orc_fill_numeric_par.argtypes = [I32, I32, V, V, V, V, V, V, V, V, V, V, C.c_int]
         for name in ("orc_spgemm_f32", "orc_spgemm_f32_omp"):
             f = getattr(o, name)
             f.restype = C.c_int
@@ -54,14 +58,29 @@ def _p(a):
     return a.ctypes.data
 
 
-def symbolic(A, B):
-    """(row_ptr int32[m+1], ops int64[m], total) of C = A*B; raises on overflow."""
+def threads_default() -> int:
+    """Host threads for the row-parallel oracle: the CPUs this process may
+    use, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def symbolic(A, B, threads=0):
+    """(row_ptr int32[m+1], ops int64[m], total) of C = A*B; raises on
+    overflow. threads > 0: the row-parallel count (same outputs)."""
     o = oracle()
     rp = np.zeros(A.rows + 1, np.int32)
     ops = np.zeros(max(A.rows, 1), np.int64)
     tot = C.c_int64(0)
-    rc = o.orc_symbolic_count(A.rows, B.cols, _p(A.row_ptr), _p(A.col_idx), _p(B.row_ptr),
-                              _p(B.col_idx), _p(rp), _p(ops), C.byref(tot))
+    if threads > 0:
+        rc = o.orc_symbolic_count_par(A.rows, B.cols, _p(A.row_ptr), _p(A.col_idx), _p(B.row_ptr),
+                                      _p(B.col_idx), _p(rp), _p(ops), C.byref(tot), threads)
+    else:
+        rc = o.orc_symbolic_count(A.rows, B.cols, _p(A.row_ptr), _p(A.col_idx), _p(B.row_ptr),
+                                  _p(B.col_idx), _p(rp), _p(ops), C.byref(tot))
     if rc != 0:
         raise OverflowError(f"oracle symbolic count status {rc}")
     return rp, ops[: A.rows], tot.value
@@ -82,6 +101,26 @@ def spgemm(A, B, f64=False):
         vd = np.zeros(max(tot, 1), np.float64)
         o.orc_numeric_f64(A.rows, B.cols, _p(A.row_ptr), _p(A.col_idx), _p(A.values),
                           _p(B.row_ptr), _p(B.col_idx), _p(B.values), _p(rp), _p(ci), _p(vd))
+        out.append(vd[:tot])
+    return tuple(out)
+
+
+def spgemm_par(A, B, threads=None, f64=True, sym=None):
+    """spgemm() over row-parallel oracle passes: (row_ptr, col_idx, values_f32
+    [, values_f64]). `sym` = a (row_ptr, ops, total) already counted."""
+    o = oracle()
+    threads = threads or threads_default()
+    rp, _, tot = sym if sym is not None else symbolic(A, B, threads=threads)
+    ci = np.zeros(max(tot, 1), np.int32)
+    va = np.zeros(max(tot, 1), np.float32)
+    vd = np.zeros(max(tot, 1), np.float64) if f64 else None
+    rc = o.orc_fill_numeric_par(A.rows, B.cols, _p(A.row_ptr), _p(A.col_idx), _p(A.values),
+                                _p(B.row_ptr), _p(B.col_idx), _p(B.values), _p(rp), _p(ci),
+                                _p(va), _p(vd) if f64 else None, threads)
+    if rc != 0:
+        raise MemoryError(f"oracle fill/numeric status {rc}")
+    out = [rp, ci[:tot], va[:tot]]
+    if f64:
         out.append(vd[:tot])
     return tuple(out)
 

@@ -220,3 +220,31 @@ def test_planted_mistake_is_caught():
     bad_ci[s], bad_ci[s + 1] = bad_ci[s + 1], bad_ci[s]
     with pytest.raises(AssertionError):
         assert_bitexact(rvsp.Csr(A.rows, B.cols, rp, bad_ci, va), rp, ci, va)
+
+
+@pytest.mark.parametrize("wl", ["rmat", "mixed", "stencil"])
+def test_row_parallel_oracle_equals_sequential(wl):
+    """The row-parallel count and fill/numeric (used at full benchmark
+    sizes) give the sequential passes' outputs bit for bit, and the FP64
+    pass equals orc_numeric_f64."""
+    from rvsp import workloads as W
+
+    A, B = W.SMALL[wl]()
+    rp, ops, tot = pyoracle.symbolic(A, B)
+    prp, pops, ptot = pyoracle.symbolic(A, B, threads=4)
+    assert tot == ptot
+    np.testing.assert_array_equal(rp, prp)
+    np.testing.assert_array_equal(ops, pops)
+    want = pyoracle.spgemm(A, B, f64=True)
+    got = pyoracle.spgemm_par(A, B, threads=4)
+    for w, g in zip(want, got):
+        np.testing.assert_array_equal(np.asarray(w).view(np.uint8), np.asarray(g).view(np.uint8))
+
+
+def test_row_parallel_count_reports_overflow():
+    # 2^15 rows, each with 2^16 distinct columns: nnz(C) = 2^31 > INT32_MAX
+    m, n = 1 << 15, 1 << 16
+    B = rvsp.Csr.make(1, n, [0, n], np.arange(n, dtype=np.int32), np.ones(n, np.float32))
+    A = rvsp.Csr.make(m, 1, np.arange(m + 1, dtype=np.int32), np.zeros(m, np.int32), np.ones(m, np.float32))
+    with pytest.raises(OverflowError):
+        pyoracle.symbolic(A, B, threads=4)
