@@ -42,3 +42,26 @@ gcc -std=c11 -O3 -fPIC -shared -w -I"$REF/tools/include" \
 mv "$OUT/librvsp_genmat_ref.so.tmp" "$OUT/librvsp_genmat_ref.so"
 rm -rf "$OUT/obj"
 echo "build_ref: wrote $OUT/librvsp_ref.so, librvsp_mtx_ref.so and librvsp_genmat_ref.so"
+# rv-sparse's own C callers of rv_sparse.h -- its two API tests and three
+# examples -- compiled unchanged from where they lie, twice each:
+#   callers/<name>.hip  against this build's include/ and linked to
+#                       rv-sparse_amd/lib/librvsparse_hip.so (the drop-in)
+#   callers/<name>.ref  against rv-sparse's include/ and linked to
+#                       _ref/librvsp_ref.so (its own scalar library)
+# Run side by side, the pair shows that a caller needs no source change.
+HIPLIB="$HERE/../rv-sparse_amd/lib"
+if [ -f "$HIPLIB/librvsparse_hip.so" ]; then
+    mkdir -p "$OUT/callers"
+    for src in "$REF/tests/test_spgemm_reference_f32.c" "$REF/tests/test_spgemm_descr_f32.c" \
+               "$REF/examples/spgemm_csr_f32.c" "$REF/examples/spgemm_csr_descr_f32.c" \
+               "$REF/examples/spgemm_csr_reuse_f32.c"; do
+        name=$(basename "$src" .c)
+        gcc -std=c11 -O2 -w -I"$HERE/../include" -I"$REF/tests" "$src" \
+            -L"$HIPLIB" -lrvsparse_hip -Wl,-rpath,'$ORIGIN/../../../rv-sparse_amd/lib' -lm \
+            -o "$OUT/callers/$name.hip"
+        gcc -std=c11 -O2 -w -I"$REF/include" -I"$REF/tests" "$src" \
+            -L"$OUT" -l:librvsp_ref.so -Wl,-rpath,'$ORIGIN/..' -lm \
+            -o "$OUT/callers/$name.ref"
+    done
+    echo "build_ref: wrote $OUT/callers (rv-sparse's tests and examples, against both libraries)"
+fi
