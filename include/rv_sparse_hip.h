@@ -75,7 +75,11 @@ typedef enum
  * pointers in A, B (and C at compute) must be device allocations that stay
  * valid until the descriptor is destroyed or re-estimated, and compute()
  * returns once the work is queued on the descriptor's stream (no host
- * synchronisation); C->row_ptr then receives a device-to-device copy. */
+ * synchronisation); C->row_ptr then receives a device-to-device copy.
+ * work_estimation checks device operands for canonical CSR on the device
+ * (RVSP_ERROR_INVALID_CSR) before anything reads through their indices;
+ * compute must be given the same row_ptr and col_idx arrays the estimation
+ * saw (RVSP_ERROR_INVALID_ARGUMENT otherwise); values may change freely. */
 rvsp_status_t rvsp_spgemm_set_memory(rvsp_spgemm_descr_t descr,
                                      rvsp_memory_t memory);
 
@@ -105,8 +109,9 @@ rvsp_status_t rvsp_spgemm_get_products(rvsp_spgemm_descr_t descr,
 /*
  * C = A * B with A, B device-resident. C receives device arrays allocated by
  * the library (owns_data = RVSP_OWNS_DEVICE); release them with
- * rvsp_csr_destroy. No canonical check is made (the data is on the device);
- * inputs must be canonical CSR. options as for rvsp_spgemm_csr, plus
+ * rvsp_csr_destroy. Operands that are not canonical CSR are found on the
+ * device and reported as RVSP_ERROR_INVALID_CSR before any product kernel
+ * runs. options as for rvsp_spgemm_csr, plus
  * algo selection through rvsp_spgemm_csr_device_algo. Synchronous.
  */
 rvsp_status_t rvsp_spgemm_csr_device(const rvsp_csr_matrix_t *A,
@@ -127,7 +132,9 @@ typedef struct rvsp_spgemm_batch *rvsp_spgemm_batch_t;
  * C[i] = A[i] * B[i] for i < count, all device-resident FP32 CSR, computed
  * as one stacked product per chunk (chunks keep each stacked C within
  * int32 indexing), so every kernel launch covers many problems. Results are
- * bit-identical to separate rvsp_spgemm_csr_device calls. C[i] receives
+ * bit-identical to separate rvsp_spgemm_csr_device calls. Every problem's
+ * operands are checked for canonical CSR on the device first
+ * (RVSP_ERROR_INVALID_CSR). C[i] receives
  * device pointers into storage owned by *batch (owns_data = 0); release it
  * with rvsp_spgemm_batch_destroy. Synchronous.
  */

@@ -39,8 +39,15 @@ void *rvsp_hip_plan_stream(const rvsp_hip_plan_t *plan);
  * Symbolic analysis of C = A * B (A m x k, B k x n) on device-resident
  * structure. Fills per-row product counts, per-row nnz(C), C's row pointer
  * (device) and the numeric bins. Blocks until nnz(C) is known.
- * Returns RVSP_ERROR_ALLOCATION_FAILED when nnz(C) > INT32_MAX.
+ * Returns RVSP_ERROR_ALLOCATION_FAILED when nnz(C) > INT32_MAX, and
+ * RVSP_ERROR_INVALID_CSR, before any kernel reads through their indices,
+ * when A or B is not canonical CSR (checked on the device unless
+ * rvsp_hip_plan_operands_checked was called since the last analysis).
  */
+/* The next rvsp_hip_plan_analyse skips its device canonical-CSR check: the
+ * caller has checked the operands on the host (rvsp_csr_check). */
+void rvsp_hip_plan_operands_checked(rvsp_hip_plan_t *plan);
+
 rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *plan, int32_t m,
                                     int32_t k, int32_t n, int32_t nnz_a,
                                     const int32_t *a_row_ptr,
@@ -162,8 +169,9 @@ rvsp_status_t rvsp_hip_alloc_async(void **ptr, size_t bytes, void *stream);
 rvsp_status_t rvsp_hip_free_async(void *ptr, void *stream);
 
 /* Batch packing (see rvsp_batch.hip). One item per stacked matrix: its
- * device arrays, its first row and first entry in the packed arrays, and
- * the shift added to its column indices. */
+ * device arrays, its first row and first entry in the packed arrays, the
+ * shift added to its column indices, and its own column count (its column
+ * indices must lie in [0, cols)). */
 typedef struct
 {
     const int32_t *row_ptr;
@@ -174,10 +182,14 @@ typedef struct
     int64_t row_off;
     int64_t nnz_off;
     int32_t col_shift;
+    int32_t cols;
 } rvsp_hip_pack_item_t;
 
 /* Writes the stacked row pointer (rows+1 entries in total), column indices
- * and values of `count` matrices into out_*. Synchronises the stream. */
+ * and values of `count` matrices into out_*. Synchronises the stream.
+ * Returns RVSP_ERROR_INVALID_CSR when an item's row pointer is not a valid
+ * CSR row pointer over its nnz or a column lies outside [0, cols) (order
+ * within rows is left to the analysis of the stacked product). */
 rvsp_status_t rvsp_hip_pack(void *stream, int32_t count, const rvsp_hip_pack_item_t *items,
                             int32_t *out_rp, int32_t *out_ci, float *out_va);
 

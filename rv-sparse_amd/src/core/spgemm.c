@@ -19,6 +19,9 @@
 #include "rvsp_hip_abi.h"
 #include "rvsp_spgemm.h"
 
+/* spgemm_descr.c */
+void rvsp_descr_operands_checked(rvsp_spgemm_descr_t d);
+
 /* spgemm_legacy.c */
 rvsp_status_t rvsp_spgemm_legacy_hip(const rvsp_csr_matrix_t *A, const rvsp_csr_matrix_t *B,
                                      rvsp_csr_matrix_t *C, rvsp_dtype_t in);
@@ -54,6 +57,7 @@ static rvsp_status_t oneshot_host(const rvsp_csr_matrix_t *A, const rvsp_csr_mat
     void *ost = rvsp_hip_oneshot_stream();
     if (s == RVSP_SUCCESS && ost)
         s = rvsp_spgemm_set_stream(d, ost);
+    rvsp_descr_operands_checked(d);
     if (s == RVSP_SUCCESS)
         s = rvsp_spgemm_work_estimation(d, A, B, &ws_bytes, &c_nnz);
     if (s != RVSP_SUCCESS)

@@ -133,7 +133,7 @@ static rvsp_status_t run_chunk(struct rvsp_spgemm_batch *batch, int32_t p0, int3
         {
             const rvsp_csr_matrix_t *M = &A[p0 + i];
             items[i] = (rvsp_hip_pack_item_t){M->row_ptr, M->col_idx, (const float *)M->values,
-                                              M->rows, M->nnz, ro, no, (int32_t)ko};
+                                              M->rows, M->nnz, ro, no, (int32_t)ko, M->cols};
             row_off[i] = ro;
             nrows[i] = M->rows;
             ro += M->rows;
@@ -149,7 +149,7 @@ static rvsp_status_t run_chunk(struct rvsp_spgemm_batch *batch, int32_t p0, int3
         {
             const rvsp_csr_matrix_t *M = &B[p0 + i];
             items[i] = (rvsp_hip_pack_item_t){M->row_ptr, M->col_idx, (const float *)M->values,
-                                              M->rows, M->nnz, ro, no, 0};
+                                              M->rows, M->nnz, ro, no, 0, M->cols};
             ro += M->rows;
             no += M->nnz;
         }

@@ -36,6 +36,9 @@ struct rvsp_spgemm_descr
 
     int32_t a_rows, a_cols, a_nnz;
     int32_t b_rows, b_cols, b_nnz;
+    /* device mode: the structure arrays the analysis read (and the kernels
+       keep reading) */
+    const int32_t *e_arp, *e_aci, *e_brp, *e_bci;
     size_t workspace_bytes;
 
     rvsp_hip_plan_t *plan;
@@ -46,7 +49,17 @@ struct rvsp_spgemm_descr
     int32_t *d_cci;
     float *d_cval;
     int b_shares_a; /* B is the same host matrix as A (C = A * A) */
+    int host_checked; /* the caller ran rvsp_csr_check on the next operands */
 };
+
+/* Tells the next host-mode work_estimation that its operands passed
+ * rvsp_csr_check already (the one-shot checks them before the descriptor
+ * exists, as rv-sparse's does), so it does not scan them again. */
+__attribute__((visibility("hidden"))) void rvsp_descr_operands_checked(rvsp_spgemm_descr_t d)
+{
+    if (d != NULL)
+        d->host_checked = 1;
+}
 
 static int algo_supported(rvsp_spgemm_algo_t algo)
 {
@@ -187,6 +200,8 @@ rvsp_status_t rvsp_spgemm_work_estimation(rvsp_spgemm_descr_t descr, const rvsp_
 {
     if (descr == NULL || workspace_bytes == NULL || c_nnz_out == NULL)
         return RVSP_ERROR_NULL_POINTER;
+    const int checked = descr->host_checked;
+    descr->host_checked = 0;
     rvsp_status_t s = check_operand(A);
     if (s != RVSP_SUCCESS)
         return s;
@@ -195,6 +210,15 @@ rvsp_status_t rvsp_spgemm_work_estimation(rvsp_spgemm_descr_t descr, const rvsp_
         return s;
     if (A->cols != B->rows)
         return RVSP_ERROR_INVALID_ARGUMENT;
+
+    /* the kernels index through the operands' structure: host operands
+       are checked here, before any upload, device operands by the analysis
+       on the device (rv-sparse leaves non-canonical input to the
+       descriptor path undefined; this build rejects it) */
+    if (descr->memory == RVSP_MEMORY_HOST && !checked &&
+        (rvsp_csr_check(A->rows, A->cols, A->row_ptr, A->col_idx, NULL) != RVSP_CSR_OK ||
+         rvsp_csr_check(B->rows, B->cols, B->row_ptr, B->col_idx, NULL) != RVSP_CSR_OK))
+        return RVSP_ERROR_INVALID_CSR;
 
     /* a new estimation replaces whatever the descriptor held */
     reset_structure(descr);
@@ -205,6 +229,7 @@ rvsp_status_t rvsp_spgemm_work_estimation(rvsp_spgemm_descr_t descr, const rvsp_
     const int32_t *arp = A->row_ptr, *aci = A->col_idx, *brp = B->row_ptr, *bci = B->col_idx;
     if (descr->memory == RVSP_MEMORY_HOST)
     {
+        rvsp_hip_plan_operands_checked(descr->plan);
         s = upload_structure(descr, A, B);
         if (s != RVSP_SUCCESS)
         {
@@ -260,6 +285,10 @@ rvsp_status_t rvsp_spgemm_work_estimation(rvsp_spgemm_descr_t descr, const rvsp_
     descr->b_rows = B->rows;
     descr->b_cols = B->cols;
     descr->b_nnz = B->nnz;
+    descr->e_arp = A->row_ptr;
+    descr->e_aci = A->col_idx;
+    descr->e_brp = B->row_ptr;
+    descr->e_bci = B->col_idx;
     rvsp_spgemm_buffer_size(B->cols, &descr->workspace_bytes);
     descr->estimated = 1;
 
@@ -296,6 +325,13 @@ rvsp_status_t rvsp_spgemm_compute(rvsp_spgemm_descr_t descr, const rvsp_csr_matr
         return RVSP_ERROR_INVALID_ARGUMENT;
     if (!algo_supported(descr->algo))
         return RVSP_ERROR_UNSUPPORTED_BACKEND;
+    /* device mode reads A's and B's structure in place through the plan,
+       which is sized by the arrays it analysed: other arrays, even equal
+       ones, need a new estimation */
+    if (descr->memory == RVSP_MEMORY_DEVICE &&
+        (A->row_ptr != descr->e_arp || A->col_idx != descr->e_aci ||
+         B->row_ptr != descr->e_brp || B->col_idx != descr->e_bci))
+        return RVSP_ERROR_INVALID_ARGUMENT;
 
     rvsp_hip_plan_t *plan = descr->plan;
     void *st = rvsp_hip_plan_stream(plan);

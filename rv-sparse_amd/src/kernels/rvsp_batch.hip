@@ -28,6 +28,7 @@ struct PackJob {
     int64_t row_off;
     int64_t nnz_off;
     int32_t col_shift;
+    int32_t cols;
     int32_t tile0;
 };
 
@@ -47,20 +48,33 @@ __device__ __forceinline__ int find_job(const PackJob *jobs, int njobs, int tile
 // One block per tile of kPackTile elements of one job. Writes row pointers
 // (shifted by the job's nnz offset, including the closing entry, which the
 // next job writes with the same value) and the entries (columns shifted).
+// Every read stays inside the job's arrays (rows + 1 pointers, nnz entries)
+// whatever they hold; a row pointer outside [0, nnz], a decreasing one, a
+// first one other than 0 or a last one other than nnz, and a column outside
+// [0, cols) set *err, so the stacked product is never analysed then.
 __global__ __launch_bounds__(256) void k_pack(const PackJob *__restrict__ jobs, int njobs,
                                               int32_t *__restrict__ orp, int32_t *__restrict__ oci,
-                                              float *__restrict__ ova)
+                                              float *__restrict__ ova, int *__restrict__ err)
 {
     const int j = find_job(jobs, njobs, blockIdx.x);
     const PackJob jb = jobs[j];
     const int64_t t0 = (int64_t)(blockIdx.x - jb.tile0) * kPackTile;
+    bool bad = false;
     for (int64_t e = t0 + threadIdx.x; e < t0 + kPackTile; e += 256) {
-        if (e <= jb.rows) orp[jb.row_off + e] = (int32_t)(jb.rp[e] + jb.nnz_off);
+        if (e <= jb.rows) {
+            const int32_t v = jb.rp[e];
+            bad |= v < 0 || v > jb.nnz || (e == 0 && v != 0) || (e == jb.rows && v != jb.nnz) ||
+                   (e < jb.rows && jb.rp[e + 1] < v);
+            orp[jb.row_off + e] = (int32_t)(v + jb.nnz_off);
+        }
         if (e < jb.nnz) {
-            oci[jb.nnz_off + e] = jb.ci[e] + jb.col_shift;
+            const int32_t c = jb.ci[e];
+            bad |= c < 0 || c >= jb.cols;
+            oci[jb.nnz_off + e] = c + jb.col_shift;
             ova[jb.nnz_off + e] = jb.va[e];
         }
     }
+    if (bad) atomicOr(err, 1);
 }
 
 struct UnpackJob {
@@ -107,22 +121,30 @@ rvsp_status_t rvsp_hip_pack(void *stream, int32_t count, const rvsp_hip_pack_ite
         const rvsp_hip_pack_item_t &it = items[i];
         const int64_t work = (int64_t)(it.rows + 1 > it.nnz ? it.rows + 1 : it.nnz);
         jobs[i] = PackJob{it.row_ptr, it.col_idx, it.values, it.rows, it.nnz,
-                          it.row_off, it.nnz_off, it.col_shift, tiles};
+                          it.row_off, it.nnz_off, it.col_shift, it.cols, tiles};
         tiles += (int)((work + kPackTile - 1) / kPackTile);
     }
-    PackJob *d = nullptr;
-    if (hipMallocAsync((void **)&d, jobs.size() * sizeof(PackJob), st) != hipSuccess)
+    // the job table, then the error word
+    const size_t table = (jobs.size() * sizeof(PackJob) + 255) & ~(size_t)255;
+    char *d = nullptr;
+    if (hipMallocAsync((void **)&d, table + 256, st) != hipSuccess)
         return RVSP_ERROR_ALLOCATION_FAILED;
+    int *derr = (int *)(d + table);
     if (hipMemcpyAsync(d, jobs.data(), jobs.size() * sizeof(PackJob), hipMemcpyHostToDevice, st) !=
-        hipSuccess) {
+            hipSuccess ||
+        hipMemsetAsync(derr, 0, 4, st) != hipSuccess) {
+        (void)hipStreamSynchronize(st);
         (void)hipFreeAsync(d, st);
         return RVSP_ERROR_UNSUPPORTED_BACKEND;
     }
-    k_pack<<<tiles, 256, 0, st>>>(d, count, out_rp, out_ci, out_va);
-    const hipError_t e = hipGetLastError();
+    k_pack<<<tiles, 256, 0, st>>>((const PackJob *)d, count, out_rp, out_ci, out_va, derr);
+    hipError_t e = hipGetLastError();
+    int herr = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&herr, derr, 4, hipMemcpyDeviceToHost, st);
     (void)hipStreamSynchronize(st);  // the host job table must outlive the copy
     (void)hipFreeAsync(d, st);
-    return e == hipSuccess ? RVSP_SUCCESS : RVSP_ERROR_UNSUPPORTED_BACKEND;
+    if (e != hipSuccess) return RVSP_ERROR_UNSUPPORTED_BACKEND;
+    return herr ? RVSP_ERROR_INVALID_CSR : RVSP_SUCCESS;
 }
 
 rvsp_status_t rvsp_hip_unpack_row_ptr(void *stream, int32_t count, const int64_t *row_off,

@@ -17,6 +17,7 @@ constexpr int kWave = 64;
 constexpr int kErrTableFull = 1;    // hash insert found no free slot
 constexpr int kErrProbeMiss = 2;    // lookup of a column C's structure lacks
 constexpr int kErrWindow = 4;       // product column outside its window
+constexpr int kErrOperand = 8;      // a device operand is not canonical CSR
 
 // Symbolic bins (per row, from product count and column span).
 enum SymBin : int {

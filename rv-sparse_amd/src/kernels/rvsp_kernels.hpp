@@ -48,6 +48,11 @@ struct BigArgs {
     int32_t *cols_out;    // numeric pass also writes the big rows' columns here (or null)
 };
 
+// Canonical-CSR check of a device operand (rvsp_validate.hip): sets
+// kErrOperand in *err on a violation; `starts` holds nnz/32 + 1 words.
+hipError_t launch_csr_validate(hipStream_t st, int32_t rows, int32_t cols, int32_t nnz,
+                               const int32_t *rp, const int32_t *ci, uint32_t *starts, int *err);
+
 // Launchable big-row window shapes (outputs, chunks); the first is the default.
 constexpr int kBigShapes[][2] = {{2048, 16}, {4096, 32}, {2048, 32}, {1024, 16}, {1024, 8}};
 constexpr int kBigShapeCount = sizeof(kBigShapes) / sizeof(kBigShapes[0]);

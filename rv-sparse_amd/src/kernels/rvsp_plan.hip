@@ -264,6 +264,7 @@ struct rvsp_hip_plan {
     AnalysisCounters *ctr = nullptr;
     int32_t *binoff = nullptr;  // [0,64) bin offsets, [64,128) scatter cursors
     int *err = nullptr;
+    bool operands_checked = false;  // the next analysis skips the device check
 
     void *bigblock = nullptr;  // big-row bitmaps, chunk counts, windows, segments
     int4 *cursors = nullptr;
@@ -378,6 +379,11 @@ void *rvsp_hip_oneshot_stream(void)
     return (void *)st;
 }
 
+void rvsp_hip_plan_operands_checked(rvsp_hip_plan_t *p)
+{
+    if (p) p->operands_checked = true;
+}
+
 const int32_t *rvsp_hip_plan_row_ptr(const rvsp_hip_plan_t *p) { return p ? p->crp : nullptr; }
 
 const int64_t *rvsp_hip_plan_op_counts(const rvsp_hip_plan_t *p) { return p ? p->ops : nullptr; }
@@ -445,6 +451,27 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     HIP_TRY(hipMemsetAsync(p->ctr, 0, sizeof(AnalysisCounters), st));
     HIP_TRY(hipMemsetAsync(p->binoff, 0, 128 * 4, st));
     HIP_TRY(hipMemsetAsync(p->err, 0, 4, st));
+
+    // ---- 0. operands the host has not checked: canonical CSR, on the
+    // device, before any kernel trusts their indices
+    const bool checked = p->operands_checked;
+    p->operands_checked = false;
+    if (!checked) {
+        const bool same = arp == brp && aci == bci && m == k && k == n && nnz_a == nnz_b;
+        const int32_t most = std::max(nnz_a, nnz_b);
+        void *starts = nullptr;
+        s = dalloc(&starts, (((size_t)most + 32) / 32) * 4, st);
+        if (s != RVSP_SUCCESS) return s;
+        hipError_t e = launch_csr_validate(st, m, k, nnz_a, arp, aci, (uint32_t *)starts, p->err);
+        if (e == hipSuccess && !same)
+            e = launch_csr_validate(st, k, n, nnz_b, brp, bci, (uint32_t *)starts, p->err);
+        int herr = 0;
+        if (e == hipSuccess) e = hipMemcpyAsync(&herr, p->err, 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        dfree(starts, st);
+        HIP_TRY(e);
+        if (herr) return RVSP_ERROR_INVALID_CSR;
+    }
 
     // learned selectors (null when none is loaded: heuristic rules)
     float margin = 1.0f, cmargin = 1.0f;

@@ -86,13 +86,19 @@ def test_checks_follow_the_legacy_wrappers():
     assert run(L, f32A, f32B, INT8, INT32)[0] == rvsp.ERROR_UNSUPPORTED_DTYPE  # wrapper :43
     a64 = with_values(A, FP64, 3)
     assert run(L, a64, a64, FP64, FP64)[0] == rvsp.ERROR_INVALID_ARGUMENT  # 30 cols x 20 rows
-    bad = A.col_idx.copy()
-    s = A.row_ptr[1]
-    if A.row_ptr[2] - s >= 2:
-        bad[s], bad[s + 1] = bad[s + 1], bad[s]
-        unsorted = Mat(A.rows, A.cols, A.row_ptr, bad, np.zeros(A.nnz), FP64)
-        b64 = with_values(B, FP64, 4)
-        assert run(L, unsorted, b64, FP64, FP64)[0] == rvsp.ERROR_INVALID_CSR
+    # a B row that repeats a column: refused on the host, before any device work
+    rep = B.col_idx.copy()
+    r = int(np.nonzero(np.diff(B.row_ptr) >= 2)[0][0])
+    rep[B.row_ptr[r] + 1] = rep[B.row_ptr[r]]
+    b_rep = Mat(B.rows, B.cols, B.row_ptr, rep, np.ones(B.nnz), FP64)
+    assert run(L, with_values(A, FP64, 3), b_rep, FP64, FP64)[0] == rvsp.ERROR_INVALID_CSR
+    # a column out of range after an order violation still counts as the
+    # range error the legacy kernels report
+    late = B.col_idx.copy()
+    late[B.row_ptr[r]], late[B.row_ptr[r] + 1] = late[B.row_ptr[r] + 1], late[B.row_ptr[r]]
+    late[-1] = B.cols
+    b_late = Mat(B.rows, B.cols, B.row_ptr, late, np.ones(B.nnz), FP64)
+    assert run(L, with_values(A, FP64, 3), b_late, FP64, FP64)[0] == rvsp.ERROR_INVALID_ARGUMENT
     oob = A.col_idx.copy()
     oob[0] = 999
     assert run(L, Mat(A.rows, A.cols, A.row_ptr, oob, np.zeros(A.nnz), FP64), with_values(B, FP64, 5),
@@ -128,6 +134,47 @@ def test_products_match_reference_legacy_kernels(din, dout):
         assert_same(got, want, f"{name} dtype {din}")
         # the hip backend name gives the same result
         assert_same(run(L, a, b, din, dout, backend=rvsp.BACKEND_HIP)[1], want, f"{name} HIP")
+
+
+def shuffled_rows(M, seed, repeat_a=False):
+    """M with every row's entries in a seeded random order (values moved
+    with their columns); repeat_a also repeats each row's first entry at
+    its end (A may hold a column twice, as the legacy kernels allow)."""
+    rng = np.random.default_rng(seed)
+    rp = [0]
+    ci, va = [], []
+    for r in range(M.rows):
+        s, e = M.row_ptr[r], M.row_ptr[r + 1]
+        order = rng.permutation(e - s) + s
+        c, v = list(M.col_idx[order]), list(M.values[order])
+        if repeat_a and e > s:
+            c.append(c[0])
+            v.append(v[0])
+        ci += c
+        va += v
+        rp.append(len(ci))
+    return rvsp.Csr(M.rows, M.cols, np.array(rp, np.int32), np.array(ci, np.int32),
+                    np.array(va, np.float32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("din,dout", [(FP64, FP64), (INT8, INT32)])
+def test_unsorted_rows_match_reference_legacy_kernels(din, dout):
+    """Unsorted A rows (with repeated columns) and unsorted B rows, which the
+    legacy kernels accept: same C bit for bit, for every row bin and the
+    typed big-row kernel."""
+    R = ref_lib()
+    L = rvsp.lib()
+    for i, (name, A, B) in enumerate(pairs()):
+        As = shuffled_rows(A, 300 + i, repeat_a=True)
+        Bs = shuffled_rows(B, 400 + i)
+        a, b = with_values(As, din, 500 + i), with_values(Bs, din, 600 + i)
+        print(f"legacy unsorted {name} dtype={din}", flush=True)
+        st, got = run(L, a, b, din, dout)
+        assert st == 0, rvsp.status_string(st)
+        st, want = run(R, a, b, din, dout)
+        assert st == 0
+        assert_same(got, want, f"unsorted {name} dtype {din}")
 
 
 @pytest.mark.gpu
