@@ -124,6 +124,37 @@ rvsp_status_t rvsp_spgemm_csr_device_algo(const rvsp_csr_matrix_t *A,
                                           rvsp_csr_matrix_t *C,
                                           rvsp_spgemm_algo_t algo);
 
+/* ---- allocation-free device one-shot --------------------------------------- */
+
+/*
+ * No counterpart in rv-sparse. For callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc): C = A * B with A,
+ * B device-resident FP32 CSR, as rvsp_spgemm_csr_device with
+ * RVSP_SPGEMM_ALGO_HIP (heuristic row bins, bit-identical results), but
+ * without any device or host allocation. Every device array the product
+ * needs besides its inputs and outputs comes from one fixed device buffer
+ * of RVSP_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds
+ * until the process ends.
+ *   C: the caller's device arrays, row_ptr with A->rows + 1 entries and
+ *      col_idx / values with room for C->nnz entries (C->nnz = capacity on
+ *      input). On success C->nnz = nnz(C), rows, cols, dtype and format are
+ *      set and owns_data = 0. When nnz(C) exceeds the capacity:
+ *      RVSP_ERROR_INVALID_ARGUMENT with C->nnz = the entries needed.
+ *   stream: a hipStream_t as void * (NULL: the default stream); the call
+ *      returns when the product is complete.
+ * RVSP_ERROR_SCRATCH_EXCEEDED when the product needs more than the fixed
+ * buffer (plan arrays grow with A's rows, B's entries and the column spans
+ * of long rows); RVSP_ERROR_INVALID_CSR for non-canonical operands.
+ * Calls on one GPU share its buffer and each overwrites what the previous
+ * one left: they are serialised within a process; work on the same device
+ * that the caller overlaps with a call must not touch the outputs.
+ */
+#define RVSP_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+rvsp_status_t rvsp_spgemm_csr_device_noalloc(const rvsp_csr_matrix_t *A,
+                                             const rvsp_csr_matrix_t *B,
+                                             rvsp_csr_matrix_t *C, void *stream);
+
 /* ---- batched products ---------------------------------------------------- */
 
 typedef struct rvsp_spgemm_batch *rvsp_spgemm_batch_t;

@@ -16,7 +16,8 @@ extern "C" {
 #endif
 
 /* Status codes. Zero is success; every error is negative.
- * Same values as rv-sparse include/rv_sparse_types.h:21-30. */
+ * Same values as rv-sparse include/rv_sparse_types.h:21-30; the last one is
+ * appended by this build (no counterpart in rv-sparse). */
 typedef enum
 {
     RVSP_SUCCESS = 0,
@@ -25,7 +26,10 @@ typedef enum
     RVSP_ERROR_INVALID_CSR = -3,
     RVSP_ERROR_UNSUPPORTED_DTYPE = -4,
     RVSP_ERROR_UNSUPPORTED_BACKEND = -5,
-    RVSP_ERROR_ALLOCATION_FAILED = -6
+    RVSP_ERROR_ALLOCATION_FAILED = -6,
+    /* the product needs more device scratch than the allocation-free entry
+       point's fixed buffer holds (rv_sparse_hip.h) */
+    RVSP_ERROR_SCRATCH_EXCEEDED = -7
 } rvsp_status_t;
 
 /* Element types (rv_sparse_types.h:32-39). Only FP32 runs on the GPU path. */

@@ -27,6 +27,7 @@ ERROR_INVALID_CSR = -3
 ERROR_UNSUPPORTED_DTYPE = -4
 ERROR_UNSUPPORTED_BACKEND = -5
 ERROR_ALLOCATION_FAILED = -6
+ERROR_SCRATCH_EXCEEDED = -7
 
 DTYPE_INT8, DTYPE_INT32, DTYPE_BF16, DTYPE_FP32, DTYPE_FP64 = range(5)
 FORMAT_CSR = 0
@@ -133,6 +134,7 @@ def _declare(lib):
         "rvsp_spgemm_get_products": (C.c_int, [V, P(I64)]),
         "rvsp_spgemm_csr_device": (C.c_int, [CSRP, CSRP, CSRP, P(Options)]),
         "rvsp_spgemm_csr_device_algo": (C.c_int, [CSRP, CSRP, CSRP, C.c_int]),
+        "rvsp_spgemm_csr_device_noalloc": (C.c_int, [CSRP, CSRP, CSRP, V]),
         "rvsp_hip_fail_next_allocs": (None, [C.c_int]),
         "rvsp_hip_set_timing": (None, [C.c_int]),
         "rvsp_hip_get_timing": (C.c_int, [P(C.c_double), P(C.c_longlong), C.c_int]),

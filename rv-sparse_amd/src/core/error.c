@@ -27,6 +27,8 @@ const char *rvsp_status_to_string(rvsp_status_t status)
         return "unsupported backend";
     case RVSP_ERROR_ALLOCATION_FAILED:
         return "allocation failed";
+    case RVSP_ERROR_SCRATCH_EXCEEDED:
+        return "scratch buffer too small";
     }
     return "unknown error";
 }

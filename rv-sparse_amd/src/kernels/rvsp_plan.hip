@@ -295,7 +295,39 @@ struct rvsp_hip_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     const void *gkey[4] = {nullptr, nullptr, nullptr, nullptr};
+
+    // arena mode (the allocation-free one-shot): the analysis and the
+    // passes carve their device arrays from [arena, arena + arena_cap)
+    // instead of allocating, and run every row bin on the caller's stream
+    char *arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    bool single_stream = false;
 };
+
+// Device arrays of a plan: carved from its arena when it has one (released
+// all at once by plan_release), else from the allocator above.
+static rvsp_status_t palloc(rvsp_hip_plan *p, void **out, size_t bytes)
+{
+    if (!p->arena) return dalloc(out, bytes, p->stream);
+    *out = nullptr;
+    const size_t need = align256(bytes ? bytes : 256);
+    if (p->arena_used > p->arena_cap || need > p->arena_cap - p->arena_used)
+        return RVSP_ERROR_SCRATCH_EXCEEDED;
+    *out = p->arena + p->arena_used;
+    p->arena_used += need;
+    return RVSP_SUCCESS;
+}
+
+static void pfree(rvsp_hip_plan *p, void *ptr)
+{
+    if (!p->arena) dfree(ptr, p->stream);
+}
+
+// Whether a pass of `work` units runs its row bins on side streams.
+static bool fan_out(const rvsp_hip_plan *p, int64_t work, int64_t min_work)
+{
+    return !p->single_stream && work >= min_work;
+}
 
 static void drop_graph(rvsp_hip_plan *p)
 {
@@ -308,9 +340,9 @@ static void drop_graph(rvsp_hip_plan *p)
 
 static void plan_release(rvsp_hip_plan *p)
 {
-    dfree(p->bcv, p->stream);
+    pfree(p, p->bcv);
     p->bcv = nullptr;
-    dfree(p->bms_spill, p->stream);
+    pfree(p, p->bms_spill);
     p->bms_spill = nullptr;
     p->bms_spill_ok = false;
     dfree(p->pmblock, p->stream);
@@ -320,10 +352,12 @@ static void plan_release(rvsp_hip_plan *p)
     p->prow = nullptr;
     p->pm_products = 0;
     p->pm_ready = p->pm_skip = false;
-    dfree(p->rowblock, p->stream);
-    dfree(p->bigblock, p->stream);
-    dfree(p->cursors, p->stream);
+    pfree(p, p->rowblock);
+    pfree(p, p->bigblock);
+    pfree(p, p->cursors);
+    p->arena_used = 0;
     p->rowblock = nullptr;
+    p->err = nullptr;  // it lived in the row block
     p->bigblock = nullptr;
     p->cursors = nullptr;
     p->big = BigArgs{};
@@ -427,7 +461,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     const size_t o_binoff = off; off = align256(off + 128 * 4);
     const size_t o_border = off; off = align256(off + kBigOrderWork * 4);
     const size_t o_err = off; off = align256(off + 4);
-    rvsp_status_t s = dalloc(&p->rowblock, off, st);
+    rvsp_status_t s = palloc(p, &p->rowblock, off);
     if (s != RVSP_SUCCESS) return s;
     char *base = (char *)p->rowblock;
     p->ops = (int64_t *)(base + o_ops);
@@ -460,7 +494,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
         const bool same = arp == brp && aci == bci && m == k && k == n && nnz_a == nnz_b;
         const int32_t most = std::max(nnz_a, nnz_b);
         void *starts = nullptr;
-        s = dalloc(&starts, (((size_t)most + 32) / 32) * 4, st);
+        s = palloc(p, &starts, (((size_t)most + 32) / 32) * 4);
         if (s != RVSP_SUCCESS) return s;
         hipError_t e = launch_csr_validate(st, m, k, nnz_a, arp, aci, (uint32_t *)starts, p->err);
         if (e == hipSuccess && !same)
@@ -468,7 +502,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
         int herr = 0;
         if (e == hipSuccess) e = hipMemcpyAsync(&herr, p->err, 4, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
-        dfree(starts, st);
+        pfree(p, starts);
         HIP_TRY(e);
         if (herr) return RVSP_ERROR_INVALID_CSR;
     }
@@ -549,7 +583,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
         const size_t o_wout = bo; bo = align256(bo + chunks * 4);
         const size_t o_segs = bo; bo = align256(bo + chunks * 16);
         const size_t o_scur = bo; bo = align256(bo + chunks * 8);
-        s = dalloc(&p->bigblock, bo, st);
+        s = palloc(p, &p->bigblock, bo);
         if (s != RVSP_SUCCESS) {
             plan_release(p);
             return s;
@@ -568,7 +602,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     // ---- 2. exact nnz(C_i)
     if (p->sym_count[SB_BMS] > 0 && !bms_spill_off()) {
         // count-pass bitmaps of the one-pass-bitmap rows, read back by the fill
-        s = dalloc((void **)&p->bms_spill, (size_t)p->sym_count[SB_BMS] * kBmsWords * 4, st);
+        s = palloc(p, (void **)&p->bms_spill, (size_t)p->sym_count[SB_BMS] * kBmsWords * 4);
         if (s != RVSP_SUCCESS) {
             plan_release(p);
             return s;
@@ -581,7 +615,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     HIP_TRY(hipMemsetAsync(p->cnt, 0, M * 4, st));
     const bool sized = sizer && p->hctr.moved > 0;
     launch_sym_count(st, sa, p->sperm, p->sym_count, p->sym_off, sized,
-                     p->products >= kFanMinProducts);
+                     fan_out(p, p->products, kFanMinProducts));
     HIP_TRY(hipGetLastError());
     if (sized) {
         // rows whose learned table overflowed go back to their heuristic
@@ -611,7 +645,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
             // nperm is free until the numeric bins are scattered
             launch_bin_scatter(st, m, p->fbin, p->binoff, p->binoff + 64, p->nperm, SB_COUNT);
             HIP_TRY(hipGetLastError());
-            launch_sym_count(st, sa, p->nperm, fcount, foff, false, p->products >= kFanMinProducts);
+            launch_sym_count(st, sa, p->nperm, fcount, foff, false, fan_out(p, p->products, kFanMinProducts));
             HIP_TRY(hipGetLastError());
             s = scatter_sym();
             if (s != RVSP_SUCCESS) return s;
@@ -635,7 +669,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->nseg = p->hctr.nseg;
     p->big.nsegs = p->nseg;
     if (p->nseg > 0) {
-        s = dalloc((void **)&p->cursors, (size_t)p->hctr.cur_total * sizeof(int4), st);
+        s = palloc(p, (void **)&p->cursors, (size_t)p->hctr.cur_total * sizeof(int4));
         if (s != RVSP_SUCCESS) {
             plan_release(p);
             return s;
@@ -663,7 +697,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     if (c_nnz) *c_nnz = p->nnzc;
     if (products) *products = p->products;
     if (p->nnzc > INT32_MAX) return RVSP_ERROR_ALLOCATION_FAILED;
-    s = dalloc((void **)&p->bcv, (size_t)nnz_b * sizeof(int2), st);
+    s = palloc(p, (void **)&p->bcv, (size_t)nnz_b * sizeof(int2));
     if (s != RVSP_SUCCESS) {
         plan_release(p);
         return s;
@@ -680,7 +714,7 @@ rvsp_status_t rvsp_hip_plan_fill(rvsp_hip_plan_t *p, int32_t *cci)
                p->crp, cci,    p->big, p->err, p->ops, p->bms_spill_ok ? p->bms_spill : nullptr};
     PhaseScope timer(p->stream, PH_FILL);
     launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off, false,
-                    p->nnzc >= kFanMinFill);
+                    fan_out(p, p->nnzc, kFanMinFill));
     HIP_TRY(hipGetLastError());
     return RVSP_SUCCESS;
 }
@@ -704,10 +738,10 @@ static void queue_numeric(rvsp_hip_plan *p, const float *aval, const float *bval
     const NumArgs a = num_args(p, aval, bval, cci, cval);
     if (p->pm_ready)
         launch_numeric_ranked(p->stream, a, p->nperm, p->num_count, p->num_off, big, p->nseg,
-                              p->prow, p->pmap, p->products >= kFanMinProducts);
+                              p->prow, p->pmap, fan_out(p, p->products, kFanMinProducts));
     else
         launch_numeric(p->stream, a, p->nperm, p->num_count, p->num_off, big, p->nseg,
-                       p->products >= kFanMinProducts);
+                       fan_out(p, p->products, kFanMinProducts));
 }
 
 rvsp_status_t rvsp_hip_plan_numeric(rvsp_hip_plan_t *p, const float *aval, const float *bval,
@@ -731,7 +765,7 @@ rvsp_status_t rvsp_hip_plan_fill_numeric(rvsp_hip_plan_t *p, const float *aval,
                    p->crp, cci,    p->big, p->err, p->ops, p->bms_spill_ok ? p->bms_spill : nullptr};
         PhaseScope timer(p->stream, PH_FILL);
         launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off, /*skip_big=*/true,
-                        p->nnzc >= kFanMinFill);
+                        fan_out(p, p->nnzc, kFanMinFill));
         HIP_TRY(hipGetLastError());
     }
     PhaseScope timer(p->stream, PH_NUMERIC);
@@ -1189,3 +1223,98 @@ void rvsp_hip_event_destroy(rvsp_hip_event_t ev)
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free device one-shot (rvsp_spgemm_csr_device_noalloc).
+//
+// Its plan arrays (per-row counts and bins, big-row bitmaps, windows and
+// cursors, B's interleaved pairs, the operand check's row marks) live in
+// `scratch`, a fixed device array of RVSP_NOALLOC_SCRATCH_BYTES that every
+// GPU the library runs on holds until the process ends. Host code never
+// passes `scratch` itself (in host code the name stands for a host-side
+// shadow, not device memory): on a device's first call its device address
+// is looked up with hipGetSymbolAddress, with that device current, and kept
+// per device; kernels and copies get that address. The host-side plan is a
+// static object per device too, so a call allocates neither device nor host
+// memory. A per-device lock serialises calls from several host threads;
+// calls on one GPU from several processes would share nothing, since each
+// process holds its own copy of the array.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kNoAllocDevices = 16;
+
+static __device__ __align__(256) unsigned char scratch[RVSP_NOALLOC_SCRATCH_BYTES];
+
+struct NoAllocSlot {
+    std::mutex mu;
+    char *scratch_ptr = nullptr;  // device address of `scratch` on this device
+    rvsp_hip_plan plan;
+};
+NoAllocSlot g_noalloc[kNoAllocDevices];
+
+} // namespace
+
+extern "C" rvsp_status_t rvsp_spgemm_csr_device_noalloc(const rvsp_csr_matrix_t *A,
+                                                        const rvsp_csr_matrix_t *B,
+                                                        rvsp_csr_matrix_t *C, void *stream)
+{
+    if (!A || !B || !C) return RVSP_ERROR_NULL_POINTER;
+    if (!A->row_ptr || !A->col_idx || !A->values || !B->row_ptr || !B->col_idx || !B->values ||
+        !C->row_ptr || !C->col_idx || !C->values)
+        return RVSP_ERROR_NULL_POINTER;
+    if (A->dtype != RVSP_DTYPE_FP32 || B->dtype != RVSP_DTYPE_FP32)
+        return RVSP_ERROR_UNSUPPORTED_DTYPE;
+    if (A->rows <= 0 || A->cols <= 0 || B->cols <= 0 || A->cols != B->rows || A->nnz < 0 ||
+        B->nnz < 0 || C->nnz < 0)
+        return RVSP_ERROR_INVALID_ARGUMENT;
+    rvsp_status_t s = device_ready();
+    if (s != RVSP_SUCCESS) return s;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocDevices) return RVSP_ERROR_UNSUPPORTED_BACKEND;
+    NoAllocSlot &slot = g_noalloc[dev];
+    std::lock_guard<std::mutex> lk(slot.mu);
+    if (!slot.scratch_ptr) {
+        void *ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return RVSP_ERROR_UNSUPPORTED_BACKEND;
+        slot.scratch_ptr = (char *)ptr;
+    }
+    rvsp_hip_plan *p = &slot.plan;
+    p->stream = (hipStream_t)stream;
+    p->own_stream = false;
+    p->arena = slot.scratch_ptr;
+    p->arena_cap = RVSP_NOALLOC_SCRATCH_BYTES;
+    p->single_stream = true;
+    int64_t nnz = 0, products = 0;
+    // heuristic row bins: the learned selectors upload their weights on
+    // first use, which allocates
+    s = rvsp_hip_plan_analyse(p, A->rows, A->cols, B->cols, A->nnz, A->row_ptr, A->col_idx, B->nnz,
+                              B->row_ptr, B->col_idx, RVSP_SELECT_HEURISTIC, &nnz, &products);
+    if (s == RVSP_SUCCESS && nnz > C->nnz) {
+        // too small an output: report the size needed
+        C->nnz = (int32_t)nnz;
+        s = RVSP_ERROR_INVALID_ARGUMENT;
+    }
+    if (s == RVSP_SUCCESS) {
+        const hipError_t e = hipMemcpyAsync(C->row_ptr, p->crp, ((size_t)A->rows + 1) * 4,
+                                            hipMemcpyDeviceToDevice, p->stream);
+        s = e == hipSuccess ? RVSP_SUCCESS : map_err(e);
+    }
+    if (s == RVSP_SUCCESS)
+        s = rvsp_hip_plan_fill_numeric(p, (const float *)A->values, (const float *)B->values,
+                                       C->col_idx, (float *)C->values);
+    // also waits for the queued work when something above failed
+    const rvsp_status_t chk = rvsp_hip_plan_check(p);
+    if (s == RVSP_SUCCESS) s = chk;
+    plan_release(p);
+    if (s != RVSP_SUCCESS) return s;
+    C->rows = A->rows;
+    C->cols = B->cols;
+    C->nnz = (int32_t)nnz;
+    C->dtype = RVSP_DTYPE_FP32;
+    C->format = RVSP_FORMAT_CSR;
+    C->owns_data = 0;
+    return RVSP_SUCCESS;
+}
