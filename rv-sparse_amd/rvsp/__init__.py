@@ -16,7 +16,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(os.path.dirname(_HERE), "lib")
-LIB_PATH = os.path.join(LIB_DIR, "librvsparse_hip.so")
+# RVSP_LIB names another build of the library (e.g. a variant being timed)
+LIB_PATH = os.environ.get("RVSP_LIB") or os.path.join(LIB_DIR, "librvsparse_hip.so")
 GEN_PATH = os.path.join(LIB_DIR, "librvsp_gen.so")
 
 # rv_sparse_types.h

@@ -480,15 +480,23 @@ __device__ __forceinline__ int2 big_piece(int cj, int ej, Acc aj, int32_t col, A
     return make_int2(cj, lane_bcast(last, nin));
 }
 
-// Up to four live A entries of a window, taken from a lane mask in lane
+// Live A entries of a window taken per memory round trip (R-MAT 2^18
+// numeric pass: 4 / 8 / 16 give 26.3 / 25.8 / 25.7 ms; the pass is bound by
+// the rate its scattered piece loads miss L2, not by loads in flight).
+#ifndef RVSP_BIG_LIVE
+#define RVSP_BIG_LIVE 8
+#endif
+constexpr int kLiveGroup = RVSP_BIG_LIVE;
+
+// Up to kLiveGroup live A entries of a window, taken from a lane mask in lane
 // (= ascending k) order, with the first 64 entries of each piece loaded
 // from `pairs` (B's pairs, or their column-blocked copy).
 template <class Acc, class Pair>
 struct PieceGroup {
     int n;
-    int j[4], c[4], e[4];
-    Acc av[4], bv[4];
-    int32_t col[4];
+    int j[kLiveGroup], c[kLiveGroup], e[kLiveGroup];
+    Acc av[kLiveGroup], bv[kLiveGroup];
+    int32_t col[kLiveGroup];
 
     __device__ __forceinline__ void take(uint64_t &mask, int4 ce, Acc a_val,
                                          const Pair *__restrict__ pairs)
@@ -496,7 +504,7 @@ struct PieceGroup {
         const int lane = threadIdx.x & 63;
         n = 0;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kLiveGroup; ++u) {
             j[u] = 0;
             if (mask) {
                 j[u] = __ffsll((unsigned long long)mask) - 1;
@@ -505,7 +513,7 @@ struct PieceGroup {
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kLiveGroup; ++u) {
             c[u] = lane_bcast(ce.x, j[u]);
             e[u] = lane_bcast(ce.y, j[u]);
             av[u] = lane_bcast(a_val, j[u]);
@@ -648,11 +656,11 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
                 int cur = ce.x, nxt = ce.z;
                 uint64_t mask = __ballot(live);
                 while (mask) {
-                    // up to four live entries: issue all their loads first
+                    // up to kLiveGroup live entries: issue all their loads first
                     PieceGroup<Acc, Pair> g0;
                     g0.take(mask, ce, av, pairs);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
+                    for (int u = 0; u < kLiveGroup; ++u) {
                         if (u < g0.n) {
                             const int2 nc = big_piece(g0.c[u], g0.e[u], g0.av[u], g0.col[u],
                                                       g0.bv[u], c0, c1, outs, W, P, acc, a.err,
