@@ -120,7 +120,10 @@ def row_subset(A: rvsp.Csr, rows) -> rvsp.Csr:
 
 
 def spot_check(A, B, c, seed=7, nrows=64):
-    """Bit-exact comparison of a seeded sample of C's rows with the oracle."""
+    """Bit-exact comparison of a seeded sample of C's rows with the oracle;
+    returns the sample's parity record, with the max and mean relative
+    error of those rows against the FP64 Gustavson oracle (entries whose
+    FP64 value is 0 left out)."""
     from oracle import pyoracle
 
     rng = np.random.default_rng(seed)
@@ -128,19 +131,59 @@ def spot_check(A, B, c, seed=7, nrows=64):
     rows = np.sort(rng.choice(nonempty, size=min(nrows, len(nonempty)), replace=False))
     _, got = device_rows(c, rows)
     S = row_subset(A, rows)
-    rp, ci, va = pyoracle.spgemm(S, B)
+    rp, ci, va, vd = pyoracle.spgemm(S, B, f64=True)
     for j, r in enumerate(rows):
         gci, gva = got[r]
         if not (np.array_equal(gci, ci[rp[j]:rp[j + 1]]) and
                 np.array_equal(gva.view(np.uint32), va[rp[j]:rp[j + 1]].view(np.uint32))):
             raise SystemExit(f"spot check failed at row {r}")
-    return len(rows)
+    return parity_record(len(rows), va, vd, "seeded rows of C, bit-exact vs the oracle")
 
 
-def cpu_baseline(A, B, target_s=10.0, seed=11):
+def parity_record(rows, va, vd, what):
+    va64 = np.asarray(va, np.float64)
+    nz = np.abs(vd) > 0
+    rel = np.abs(va64[nz] - vd[nz]) / np.abs(vd[nz])
+    return {"checked": what, "rows": int(rows), "entries": int(len(vd)), "bit_exact": True,
+            "max_rel_err_f64": float(rel.max()) if rel.size else 0.0,
+            "mean_rel_err_f64": float(rel.mean()) if rel.size else 0.0}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def timed_cpu(run, warmups=3, runs=10):
+    """BASELINE.md's CPU protocol: pinned to one host core, `warmups` untimed
+    runs, then the median of `runs` timed runs. Returns (median s, core)."""
+    old = os.sched_getaffinity(0)
+    core = min(old)
+    os.sched_setaffinity(0, {core})
+    try:
+        for _ in range(warmups):
+            run()
+        times = [run() for _ in range(runs)]
+    finally:
+        os.sched_setaffinity(0, old)
+    return float(np.median(times)), core
+
+
+CPU_BUILD = ("rv-sparse's sources, gcc -std=c11 -O3 -flto -march=x86-64-v3 (rv-sparse's Makefile "
+             "uses -march=native; the library is built in another container than the host it runs on)")
+
+
+def cpu_baseline(A, B, target_s=1.0, seed=11):
     """rv-sparse's scalar one-shot (compiled from its sources, or this
     project's restatement if that build is absent) on a seeded row sample of
-    the same workload, 1 thread."""
+    the same workload sized to about target_s per run: 1 thread pinned to a
+    core, 3 warm-ups, median of 10 runs."""
     from oracle import pyoracle
 
     kind = "reference" if pyoracle.ref_available() else "port"
@@ -155,49 +198,63 @@ def cpu_baseline(A, B, target_s=10.0, seed=11):
     # calibrate on a small sample, then size the timed sample to ~target_s
     frac = 0.002
     S = row_subset(A, np.sort(perm[: max(1, int(A.rows * frac))]))
-    prods = int(pyoracle.symbolic(S, B)[1].sum())
     t = run(S)
-    rate = prods / max(t, 1e-9)
     want = min(1.0, frac * (target_s / max(t, 1e-6)))
     S = row_subset(A, np.sort(perm[: max(1, int(A.rows * want))]))
     prods = int(pyoracle.symbolic(S, B)[1].sum())
-    times = [run(S)]
-    t = min(times)
+    t, core = timed_cpu(lambda: run(S))
     return {
         "value": round(2 * prods / t / 1e9, 4),
         "unit": "GFLOP/s",
         "cores": 1,
         "kind": kind,
+        "cpu_model": cpu_model(),
+        "build": CPU_BUILD if kind == "reference" else "oracle/oracle.c, gcc -O3 -march=x86-64-v3",
+        "protocol": f"pinned to core {core}; 3 warm-ups, median of 10 runs",
         "sample": (f"{S.rows} of {A.rows} rows (seeded random), {prods} products, full one-shot "
-                   f"(count, C alloc, fill+sort, numeric): {t:.2f} s"),
+                   f"(count, C alloc, fill+sort, numeric): median {t:.3f} s"),
     }
 
 
-def cpu_baseline_batch(pairs, target_s=10.0, seed=11):
-    """rv-sparse's scalar one-shot over whole problems of the batch, in a
-    seeded order, until about target_s of CPU time has run, 1 thread."""
+def cpu_baseline_batch(pairs, target_s=1.0, seed=11):
+    """rv-sparse's scalar one-shot over whole problems of the batch, a seeded
+    subset of about target_s of CPU time per run: 1 thread pinned to a core,
+    3 warm-ups, median of 10 runs of the subset."""
     from oracle import pyoracle
 
     kind = "reference" if pyoracle.ref_available() else "port"
-    t, prods, n = 0.0, 0, 0
+
+    def one(A, B):
+        return pyoracle.ref_spgemm(A, B, timed=True)[-1] if kind == "reference" else pyoracle.time_oneshot(A, B)
+
+    subset, t, prods = [], 0.0, 0
     for i in np.random.default_rng(seed).permutation(len(pairs)):
         A, B = pairs[i]
-        t += pyoracle.ref_spgemm(A, B, timed=True)[-1] if kind == "reference" else pyoracle.time_oneshot(A, B)
+        t += one(A, B)
         prods += int(np.diff(B.row_ptr)[A.col_idx].sum())
-        n += 1
+        subset.append(i)
         if t >= target_s:
             break
+    med, core = timed_cpu(lambda: sum(one(*pairs[i]) for i in subset))
     return {
-        "value": round(2 * prods / t / 1e9, 4),
+        "value": round(2 * prods / med / 1e9, 4),
         "unit": "GFLOP/s",
         "cores": 1,
         "kind": kind,
-        "sample": (f"{n} of {len(pairs)} problems (seeded order), {prods} products, full one-shot "
-                   f"each (count, C alloc, fill+sort, numeric): {t:.2f} s"),
+        "cpu_model": cpu_model(),
+        "build": CPU_BUILD if kind == "reference" else "oracle/oracle.c, gcc -O3 -march=x86-64-v3",
+        "protocol": f"pinned to core {core}; 3 warm-ups, median of 10 runs",
+        "sample": (f"{len(subset)} of {len(pairs)} problems (seeded order), {prods} products, full "
+                   f"one-shot each (count, C alloc, fill+sort, numeric): median {med:.3f} s per run"),
     }
 
 
 ALGOS = {"hip": rvsp.ALGO_HIP, "learned": rvsp.ALGO_HIP_LEARNED}
+
+
+TRAFFIC_KIND = ("L2-miss bytes of one numeric pass (rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, "
+                "scaled by a device copy of known size; MALL hits included), read from the committed "
+                "profiles/traffic_<workload>.md")
 
 
 def measured_traffic(workload):
@@ -398,9 +455,10 @@ def main():
 
     c = step()
     nnz_c = c.nnz
+    parity = None
     if not args.no_check:
-        n = spot_check(A, B, c)
-        log(f"spot check: {n} rows bit-exact vs oracle")
+        parity = spot_check(A, B, c)
+        log(f"spot check: {parity['rows']} rows bit-exact vs oracle")
     L.rvsp_csr_destroy(C.byref(c))
     d = rvsp.Descr()
     d.set_memory(rvsp.MEMORY_DEVICE)
@@ -481,9 +539,11 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic,
+            "traffic_kind": TRAFFIC_KIND,
             "algorithmic_bytes": alg_bytes,
             "numeric_ms": round(num_avg_ms, 4),
         },
+        "parity": parity,
         "beside_roofline": beside_roofline(products, traffic, num_avg_ms),
         "phases_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in phases.items()},
         "step_roofline_frac": round(step_gbps / HBM_PEAK_GBPS, 4),
@@ -538,6 +598,7 @@ def bench_reuse(args, n_sets=16):
     sets = [rvsp.DeviceBuffer.from_array(rvsp.fill_values(A.nnz, 1000 + i, normal=True))
             for i in range(n_sets)]
     out = {}
+    out_parity = None
     for graph in (False, True):
         d = rvsp.Descr()
         d.set_memory(rvsp.MEMORY_DEVICE)
@@ -575,8 +636,9 @@ def bench_reuse(args, n_sets=16):
             # last pass used set (steps-1) % n_sets: check a sample of rows
             Ah = rvsp.Csr(A.rows, A.cols, A.row_ptr, A.col_idx,
                           rvsp.fill_values(A.nnz, 1000 + (args.steps - 1) % n_sets, normal=True))
-            n = spot_check(Ah, Ah, c)
-            log(f"graph={graph}: spot check {n} rows bit-exact")
+            par = spot_check(Ah, Ah, c)
+            log(f"graph={graph}: spot check {par['rows']} rows bit-exact")
+            out_parity = par
         d.close()
     ms_graph, num_graph = out[True]
     ms_call, num_call = out[False]
@@ -591,8 +653,9 @@ def bench_reuse(args, n_sets=16):
     rec["roofline"] = {"bound": "hbm", "kernel": "numeric pass (hipGraph replay)",
                        "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                        "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                       "traffic": measured_traffic(args.workload),
+                       "traffic": measured_traffic(args.workload), "traffic_kind": TRAFFIC_KIND,
                        "algorithmic_bytes": alg_bytes, "numeric_ms": round(num_graph, 4)}
+    rec["parity"] = out_parity
     rec["beside_roofline"] = beside_roofline(products, rec["roofline"]["traffic"], num_graph)
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline(A, A)
     print(json.dumps(rec), flush=True)
@@ -621,13 +684,25 @@ def bench_batched(args):
         _, ch, nnzs = rvsp.spgemm_batched(dA, dB, algo=algo, fetch=False, with_nnz=True)
         return ch, nnzs
 
+    parity = None
     if not args.no_check:
+        from oracle import pyoracle
         got, chunks = rvsp.spgemm_batched(dA[:8], dB[:8], algo=algo)[:2]
+        va_all, vd_all, rows = [], [], 0
         for i in range(8):
             one = rvsp.spgemm_device(dA[i], dB[i], algo=algo)
             assert np.array_equal(got[i].col_idx, one.col_idx)
             assert np.array_equal(got[i].values.view(np.uint32), one.values.view(np.uint32))
-        log("batched vs separate: first 8 problems bit-identical")
+            rp, ci, va, vd = pyoracle.spgemm(pairs[i][0], pairs[i][1], f64=True)
+            if not (np.array_equal(got[i].col_idx, ci) and
+                    np.array_equal(got[i].values.view(np.uint32), va.view(np.uint32))):
+                raise SystemExit(f"batched problem {i} differs from the oracle")
+            va_all.append(va)
+            vd_all.append(vd)
+            rows += pairs[i][0].rows
+        parity = parity_record(rows, np.concatenate(va_all), np.concatenate(vd_all),
+                               "first 8 problems: batched = separate calls = oracle, bit-exact")
+        log("batched vs separate vs oracle: first 8 problems bit-identical")
     for _ in range(max(args.warmup, 1)):
         _, nnzs = batched()
         separate()
@@ -663,8 +738,9 @@ def bench_batched(args):
     rec["roofline"] = {"bound": "hbm", "kernel": "numeric pass of every chunk (stacked problems)",
                        "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                        "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                       "traffic": measured_traffic(args.workload),
+                       "traffic": measured_traffic(args.workload), "traffic_kind": TRAFFIC_KIND,
                        "algorithmic_bytes": alg_bytes, "numeric_ms": round(num_ms, 4)}
+    rec["parity"] = parity
     rec["beside_roofline"] = beside_roofline(products, rec["roofline"]["traffic"], num_ms)
     rec["phases_ms_per_batch"] = {k: round(v[0] / max(args.steps, 1), 4) for k, v in ph.items()}
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline_batch(pairs)

@@ -154,11 +154,12 @@ rvsp_status_t rvsp_hip_plan_time_bin(rvsp_hip_plan_t *plan, int32_t bin, const i
                                      const float *b_values, const int32_t *c_col_idx,
                                      float *c_values, int32_t reps, float *ms);
 
-/* The stream the library's one-shot and batched calls run on: created on
- * first use and kept for the life of the process, so the stream-ordered
- * pool hands each call the memory the previous one freed (a fresh stream
- * per call can make every call map new memory). Calls on it from several
- * host threads are serialised. NULL if no stream could be created. */
+/* The stream the calling host thread's one-shot and batched calls run on:
+ * one per thread, created on its first use and kept for the thread's
+ * lifetime, so the library's block cache hands each call the memory the
+ * thread's previous call freed (a fresh stream per call can make every call
+ * map new memory). Threads do not queue behind each other's products. NULL
+ * if no stream could be created. */
 void *rvsp_hip_oneshot_stream(void);
 
 /* Stream helpers used by the host core. */

@@ -4,8 +4,9 @@
 # may serve as the timed CPU baseline. Only the reference's C sources are
 # compiled (no reference build system, no generated code, no stand-ins); the
 # three RISC-V Vector accumulators are left out exactly as rv-sparse's
-# Makefile does for a non-vector -march. -march=x86-64-v3 instead of the
-# reference's -march=native keeps the library runnable on any AVX2 host.
+# Makefile does for a non-vector -march. Its flags (-std=c11 -O3 -flto),
+# with -march=x86-64-v3 instead of -march=native: the library is built here
+# and runs on another host, so it targets any AVX2 host.
 # Skips quietly (exit 0) when the reference tree is absent.
 set -euo pipefail
 REF=${RVSP_REFERENCE_DIR:-/root/reference}
@@ -16,7 +17,7 @@ if [ ! -d "$REF/src" ]; then
     exit 0
 fi
 mkdir -p "$OUT/obj"
-CFLAGS="-std=c11 -O3 -march=x86-64-v3 -fPIC -I$REF/include -w"
+CFLAGS="-std=c11 -O3 -flto -march=x86-64-v3 -fPIC -I$REF/include -w"
 objs=()
 while IFS= read -r src; do
     case "$src" in
@@ -26,7 +27,7 @@ while IFS= read -r src; do
     gcc $CFLAGS -c "$src" -o "$obj"
     objs+=("$obj")
 done < <(find "$REF/src" -name '*.c' | sort)
-gcc -shared -Wl,-Bsymbolic -o "$OUT/librvsp_ref.so.tmp" "${objs[@]}" -lm
+gcc -O3 -flto -march=x86-64-v3 -shared -Wl,-Bsymbolic -o "$OUT/librvsp_ref.so.tmp" "${objs[@]}" -lm
 mv "$OUT/librvsp_ref.so.tmp" "$OUT/librvsp_ref.so"
 # rv-sparse's Matrix Market reader (tools/src: the formatter and its vector),
 # the checker of rvsp_mtx_load

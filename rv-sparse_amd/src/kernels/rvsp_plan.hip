@@ -72,19 +72,25 @@ rvsp_status_t device_ready()
     return RVSP_SUCCESS;
 }
 
-// Blocks freed on the library's one-shot stream, kept for later calls on
-// that stream. Stream order makes the reuse safe without events: every
-// earlier use of a block was queued on the same stream before its free. The
-// stream-ordered pool alone does not keep them: at a stream synchronisation
-// the runtime hands most freed blocks back to the OS whatever the release
-// threshold, and re-mapping C's arrays and the big-row scratch each call
-// costs about a millisecond per few GB, with occasional 50-150 ms stalls.
+// Blocks freed on a one-shot stream (each host thread has its own, see
+// rvsp_hip_oneshot_stream), kept for later calls on that same stream.
+// Stream order makes the reuse safe without events: every earlier use of a
+// block was queued on the stream before its free, so idle blocks are kept
+// per stream and only handed out on it. The stream-ordered pool alone does
+// not keep them: at a stream synchronisation the runtime hands most freed
+// blocks back to the OS whatever the release threshold, and re-mapping C's
+// arrays and the big-row scratch each call costs about a millisecond per
+// few GB, with occasional 50-150 ms stalls.
+struct BlockInfo {
+    size_t size;
+    hipStream_t stream;  // the one-shot stream it was allocated on
+};
 struct BlockCache {
     std::mutex mu;
-    std::multimap<size_t, void *> idle;        // size -> idle block
-    std::unordered_map<void *, size_t> sizes;  // every block the cache handed out
+    std::unordered_map<hipStream_t, std::multimap<size_t, void *>> idle;  // per stream
+    std::unordered_map<void *, BlockInfo> blocks;  // every block the cache handed out
     size_t idle_bytes = 0;
-    size_t limit = 0;  // idle bytes kept at most: a quarter of device memory
+    size_t limit = 0;  // idle bytes kept at most (all streams): a quarter of device memory
 };
 BlockCache g_cache;
 
@@ -94,16 +100,25 @@ size_t cache_round(size_t bytes)
     return (bytes + g - 1) / g * g;
 }
 
-// hands every idle block back to the pool (caller holds the lock)
+// hands a stream's idle blocks back to the pool (caller holds the lock)
 void cache_flush_locked(hipStream_t st)
 {
-    for (auto &e : g_cache.idle) {
-        g_cache.sizes.erase(e.second);
+    auto it = g_cache.idle.find(st);
+    if (it == g_cache.idle.end()) return;
+    for (auto &e : it->second) {
+        g_cache.blocks.erase(e.second);
+        g_cache.idle_bytes -= e.first;
         (void)hipFreeAsync(e.second, st);
     }
-    g_cache.idle.clear();
-    g_cache.idle_bytes = 0;
+    it->second.clear();
 }
+
+// whether st is the calling thread's one-shot stream
+bool is_oneshot(hipStream_t st) { return st && st == (hipStream_t)rvsp_hip_oneshot_stream(); }
+
+// The calling thread's stream for synchronous helpers (never the legacy
+// null stream, whose operations fail while another thread captures).
+hipStream_t thread_stream() { return (hipStream_t)rvsp_hip_oneshot_stream(); }
 
 rvsp_status_t dalloc(void **p, size_t bytes, hipStream_t st)
 {
@@ -113,7 +128,7 @@ rvsp_status_t dalloc(void **p, size_t bytes, hipStream_t st)
         return RVSP_ERROR_ALLOCATION_FAILED;
     }
     if (bytes == 0) bytes = 256;
-    if (st != (hipStream_t)rvsp_hip_oneshot_stream()) {
+    if (!is_oneshot(st)) {
         HIP_TRY(hipMallocAsync(p, bytes, st));
         return RVSP_SUCCESS;
     }
@@ -123,19 +138,25 @@ rvsp_status_t dalloc(void **p, size_t bytes, hipStream_t st)
         size_t fr = 0, tot = 0;
         g_cache.limit = hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 4 : ((size_t)1 << 32);
     }
-    // best fit, wasting at most an eighth of the block
-    auto it = g_cache.idle.lower_bound(bytes);
-    if (it != g_cache.idle.end() && it->first - bytes <= it->first / 8) {
+    // best fit among this stream's idle blocks, wasting at most an eighth
+    auto &idle = g_cache.idle[st];
+    auto it = idle.lower_bound(bytes);
+    if (it != idle.end() && it->first - bytes <= it->first / 8) {
         *p = it->second;
         g_cache.idle_bytes -= it->first;
-        g_cache.idle.erase(it);
+        idle.erase(it);
         return RVSP_SUCCESS;
     }
     hipError_t e = hipMallocAsync(p, bytes, st);
-    if (e == hipErrorOutOfMemory && !g_cache.idle.empty()) {
+    if (e == hipErrorOutOfMemory && g_cache.idle_bytes > 0) {
         (void)hipGetLastError();  // recovered below: not a launch error
-        cache_flush_locked(st);
-        (void)hipStreamSynchronize(st);
+        // every thread's idle blocks go back to the pool, each freed on the
+        // stream that last used it, which is then drained
+        for (auto &kv : g_cache.idle) {
+            if (kv.second.empty()) continue;
+            cache_flush_locked(kv.first);
+            (void)hipStreamSynchronize(kv.first);
+        }
         e = hipMallocAsync(p, bytes, st);
     }
     if (e != hipSuccess) {
@@ -143,7 +164,7 @@ rvsp_status_t dalloc(void **p, size_t bytes, hipStream_t st)
         *p = nullptr;
         return map_err(e);
     }
-    g_cache.sizes[*p] = bytes;
+    g_cache.blocks[*p] = BlockInfo{bytes, st};
     return RVSP_SUCCESS;
 }
 
@@ -152,22 +173,23 @@ void dfree(void *p, hipStream_t st)
     if (!p) return;
     {
         std::lock_guard<std::mutex> lk(g_cache.mu);
-        auto it = g_cache.sizes.find(p);
-        if (it != g_cache.sizes.end()) {
-            if (st == (hipStream_t)rvsp_hip_oneshot_stream()) {
-                g_cache.idle.emplace(it->second, p);
-                g_cache.idle_bytes += it->second;
-                // over the limit: the smallest idle blocks go back first
-                while (g_cache.idle_bytes > g_cache.limit && !g_cache.idle.empty()) {
-                    auto v = g_cache.idle.begin();
+        auto it = g_cache.blocks.find(p);
+        if (it != g_cache.blocks.end()) {
+            if (st == it->second.stream && is_oneshot(st)) {
+                auto &idle = g_cache.idle[st];
+                idle.emplace(it->second.size, p);
+                g_cache.idle_bytes += it->second.size;
+                // over the limit: this stream's smallest idle blocks go back first
+                while (g_cache.idle_bytes > g_cache.limit && !idle.empty()) {
+                    auto v = idle.begin();
                     g_cache.idle_bytes -= v->first;
-                    g_cache.sizes.erase(v->second);
+                    g_cache.blocks.erase(v->second);
                     (void)hipFreeAsync(v->second, st);
-                    g_cache.idle.erase(v);
+                    idle.erase(v);
                 }
                 return;
             }
-            g_cache.sizes.erase(it);  // freed on another stream: the pool takes it
+            g_cache.blocks.erase(it);  // freed elsewhere: the pool takes it
         }
     }
     (void)hipFreeAsync(p, st);
@@ -406,9 +428,9 @@ void *rvsp_hip_plan_stream(const rvsp_hip_plan_t *p) { return p ? (void *)p->str
 
 void *rvsp_hip_oneshot_stream(void)
 {
-    static std::mutex mu;
-    static hipStream_t st = nullptr;
-    std::lock_guard<std::mutex> lk(mu);
+    // one per host thread (kept for the thread's lifetime), so one-shots
+    // and batches of different threads do not queue behind each other
+    thread_local hipStream_t st = nullptr;
     if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
     return (void *)st;
 }
@@ -690,9 +712,9 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     HIP_TRY(hipMemsetAsync(p->binoff + 64, 0, 64 * 4, st));
     launch_bin_scatter(st, m, p->nbin, p->binoff, p->binoff + 64, p->nperm, NB_COUNT);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(st));
     int herr = 0;
-    HIP_TRY(hipMemcpy(&herr, p->err, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(&herr, p->err, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     if (herr) return RVSP_ERROR_INVALID_CSR;
     if (c_nnz) *c_nnz = p->nnzc;
     if (products) *products = p->products;
@@ -931,7 +953,8 @@ rvsp_status_t rvsp_hip_plan_check(rvsp_hip_plan_t *p)
     HIP_TRY(hipStreamSynchronize(p->stream));
     if (!p->err) return RVSP_SUCCESS;
     int herr = 0;
-    HIP_TRY(hipMemcpy(&herr, p->err, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(&herr, p->err, 4, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
     return herr ? RVSP_ERROR_INVALID_CSR : RVSP_SUCCESS;
 }
 
@@ -1031,7 +1054,8 @@ rvsp_status_t rvsp_hip_plan_time_bin(rvsp_hip_plan_t *p, int32_t bin, const int3
     (void)hipEventDestroy(e1);
     if (s != RVSP_SUCCESS) return s;
     int herr = 0;
-    HIP_TRY(hipMemcpy(&herr, p->err, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(&herr, p->err, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipMemsetAsync(p->err, 0, 4, st));
     return herr ? RVSP_ERROR_INVALID_ARGUMENT : RVSP_SUCCESS;
 }
@@ -1126,45 +1150,55 @@ rvsp_status_t rvsp_hip_malloc(void **ptr, size_t bytes)
         --g_fail_allocs;
         return RVSP_ERROR_ALLOCATION_FAILED;
     }
-    HIP_TRY(hipMalloc(ptr, bytes ? bytes : 256));
+    // stream-ordered on this thread's stream, never on the legacy null
+    // stream: a copy there fails while any other thread captures a graph
+    const hipStream_t ts = thread_stream();
+    HIP_TRY(hipMallocAsync(ptr, bytes ? bytes : 256, ts));
+    HIP_TRY(hipStreamSynchronize(ts));
     return RVSP_SUCCESS;
 }
 
 rvsp_status_t rvsp_hip_free(void *ptr)
 {
-    if (ptr) HIP_TRY(hipFree(ptr));
+    if (!ptr) return RVSP_SUCCESS;
+    const hipStream_t ts = thread_stream();
+    HIP_TRY(hipFreeAsync(ptr, ts));
+    HIP_TRY(hipStreamSynchronize(ts));
+    return RVSP_SUCCESS;
+}
+
+static rvsp_status_t sync_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    if (bytes == 0) return RVSP_SUCCESS;
+    if (!dst || !src) return RVSP_ERROR_NULL_POINTER;
+    const hipStream_t ts = thread_stream();
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, kind, ts));
+    HIP_TRY(hipStreamSynchronize(ts));
     return RVSP_SUCCESS;
 }
 
 rvsp_status_t rvsp_hip_memcpy_h2d(void *dst, const void *src, size_t bytes)
 {
-    if (bytes == 0) return RVSP_SUCCESS;
-    if (!dst || !src) return RVSP_ERROR_NULL_POINTER;
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
-    return RVSP_SUCCESS;
+    return sync_copy(dst, src, bytes, hipMemcpyHostToDevice);
 }
 
 rvsp_status_t rvsp_hip_memcpy_d2h(void *dst, const void *src, size_t bytes)
 {
-    if (bytes == 0) return RVSP_SUCCESS;
-    if (!dst || !src) return RVSP_ERROR_NULL_POINTER;
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
-    return RVSP_SUCCESS;
+    return sync_copy(dst, src, bytes, hipMemcpyDeviceToHost);
 }
 
 rvsp_status_t rvsp_hip_memcpy_d2d(void *dst, const void *src, size_t bytes)
 {
-    if (bytes == 0) return RVSP_SUCCESS;
-    if (!dst || !src) return RVSP_ERROR_NULL_POINTER;
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
-    return RVSP_SUCCESS;
+    return sync_copy(dst, src, bytes, hipMemcpyDeviceToDevice);
 }
 
 rvsp_status_t rvsp_hip_memset(void *dst, int value, size_t bytes)
 {
     if (bytes == 0) return RVSP_SUCCESS;
     if (!dst) return RVSP_ERROR_NULL_POINTER;
-    HIP_TRY(hipMemset(dst, value, bytes));
+    const hipStream_t ts = thread_stream();
+    HIP_TRY(hipMemsetAsync(dst, value, bytes, ts));
+    HIP_TRY(hipStreamSynchronize(ts));
     return RVSP_SUCCESS;
 }
 

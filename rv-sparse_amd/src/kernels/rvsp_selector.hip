@@ -355,13 +355,28 @@ SelectorSlot g_slots[2] = {{Sizer::NP}, {Chooser::NP}};
 
 bool valid_which(int which) { return which == RVSP_SELECTOR_SIZER || which == RVSP_SELECTOR_CHOOSER; }
 
+// Selector work runs on the calling thread's stream, never on the legacy
+// null stream, whose operations fail while another thread captures a graph.
+hipStream_t sel_stream() { return (hipStream_t)rvsp_hip_oneshot_stream(); }
+hipError_t sel_malloc(void **p, size_t bytes) { return hipMallocAsync(p, bytes, sel_stream()); }
+void sel_free(void *p)
+{
+    if (p) (void)hipFreeAsync(p, sel_stream());
+}
+hipError_t sel_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, sel_stream());
+    return e == hipSuccess ? hipStreamSynchronize(sel_stream()) : e;
+}
+hipError_t sel_zero(void *p, size_t bytes) { return hipMemsetAsync(p, 0, bytes, sel_stream()); }
+
 rvsp_status_t slot_set(SelectorSlot &s, const float *params, float margin)
 {
     if (!s.dev && hipMalloc((void **)&s.dev, (size_t)s.np * 4) != hipSuccess) {
         s.dev = nullptr;
         return RVSP_ERROR_ALLOCATION_FAILED;
     }
-    if (hipMemcpy(s.dev, params, (size_t)s.np * 4, hipMemcpyHostToDevice) != hipSuccess)
+    if (sel_copy(s.dev, params, (size_t)s.np * 4, hipMemcpyHostToDevice) != hipSuccess)
         return RVSP_ERROR_UNSUPPORTED_BACKEND;
     s.host.assign(params, params + s.np);
     s.margin = margin;
@@ -402,7 +417,7 @@ void launch_grad(const float *dp, const float *feats, const float *target, const
                  int n, float *part)
 {
     const int nb = (n + TB - 1) / TB;
-    k_sel_grad<Net, NF, NO><<<nb, TB>>>(dp, feats, target, idx, n, 1.0f / (float)n, part);
+    k_sel_grad<Net, NF, NO><<<nb, TB, 0, sel_stream()>>>(dp, feats, target, idx, n, 1.0f / (float)n, part);
 }
 
 void grad_launch(int which, const float *dp, const float *feats, const float *target,
@@ -418,9 +433,9 @@ void reduce_launch(int which, const float *part, int nb, float *g, float *dp, fl
                    float lr, float b1t, float b2t, int apply)
 {
     if (which == RVSP_SELECTOR_SIZER)
-        k_sel_reduce_adam<Sizer::NP><<<1, 512>>>(part, nb, g, dp, m1, m2, lr, b1t, b2t, apply);
+        k_sel_reduce_adam<Sizer::NP><<<1, 512, 0, sel_stream()>>>(part, nb, g, dp, m1, m2, lr, b1t, b2t, apply);
     else
-        k_sel_reduce_adam<Chooser::NP><<<1, 512>>>(part, nb, g, dp, m1, m2, lr, b1t, b2t, apply);
+        k_sel_reduce_adam<Chooser::NP><<<1, 512, 0, sel_stream()>>>(part, nb, g, dp, m1, m2, lr, b1t, b2t, apply);
 }
 
 } // namespace
@@ -505,19 +520,19 @@ rvsp_status_t rvsp_selector_predict(int32_t which, const float *feats, int32_t n
     if (n == 0) return RVSP_SUCCESS;
     const int np = g_slots[which].np;
     float *dp = nullptr;
-    if (hipMalloc((void **)&dp, (size_t)np * 4) != hipSuccess) return RVSP_ERROR_ALLOCATION_FAILED;
+    if (sel_malloc((void **)&dp, (size_t)np * 4) != hipSuccess) return RVSP_ERROR_ALLOCATION_FAILED;
     rvsp_status_t s = RVSP_SUCCESS;
-    if (hipMemcpy(dp, params_host, (size_t)np * 4, hipMemcpyHostToDevice) != hipSuccess)
+    if (sel_copy(dp, params_host, (size_t)np * 4, hipMemcpyHostToDevice) != hipSuccess)
         s = RVSP_ERROR_UNSUPPORTED_BACKEND;
     if (s == RVSP_SUCCESS) {
         const int grid = (n + 255) / 256;
         if (which == RVSP_SELECTOR_SIZER)
-            k_sel_predict<Sizer, 4, 1><<<grid, 256>>>(dp, feats, n, pred);
+            k_sel_predict<Sizer, 4, 1><<<grid, 256, 0, sel_stream()>>>(dp, feats, n, pred);
         else
-            k_sel_predict<Chooser, 5, 8><<<grid, 256>>>(dp, feats, n, pred);
-        if (hipDeviceSynchronize() != hipSuccess) s = RVSP_ERROR_UNSUPPORTED_BACKEND;
+            k_sel_predict<Chooser, 5, 8><<<grid, 256, 0, sel_stream()>>>(dp, feats, n, pred);
+        if (hipStreamSynchronize(sel_stream()) != hipSuccess) s = RVSP_ERROR_UNSUPPORTED_BACKEND;
     }
-    (void)hipFree(dp);
+    sel_free(dp);
     return s;
 }
 
@@ -532,27 +547,27 @@ rvsp_status_t rvsp_selector_grad(int32_t which, const float *feats, const float 
     const int nb = (n + TB - 1) / TB;
     float *dp = nullptr, *part = nullptr, *g = nullptr;
     rvsp_status_t s = RVSP_SUCCESS;
-    if (hipMalloc((void **)&dp, (size_t)np * 4) != hipSuccess ||
-        hipMalloc((void **)&part, (size_t)nb * (np + 2) * 4) != hipSuccess ||
-        hipMalloc((void **)&g, (size_t)(np + 2) * 4) != hipSuccess)
+    if (sel_malloc((void **)&dp, (size_t)np * 4) != hipSuccess ||
+        sel_malloc((void **)&part, (size_t)nb * (np + 2) * 4) != hipSuccess ||
+        sel_malloc((void **)&g, (size_t)(np + 2) * 4) != hipSuccess)
         s = RVSP_ERROR_ALLOCATION_FAILED;
     if (s == RVSP_SUCCESS &&
-        hipMemcpy(dp, params_host, (size_t)np * 4, hipMemcpyHostToDevice) != hipSuccess)
+        sel_copy(dp, params_host, (size_t)np * 4, hipMemcpyHostToDevice) != hipSuccess)
         s = RVSP_ERROR_UNSUPPORTED_BACKEND;
     if (s == RVSP_SUCCESS) {
         grad_launch(which, dp, feats, target, nullptr, n, part);
         reduce_launch(which, part, nb, g, nullptr, nullptr, nullptr, 0.0f, 0.0f, 0.0f, 0);
         std::vector<float> hg((size_t)np + 2);
-        if (hipMemcpy(hg.data(), g, hg.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+        if (sel_copy(hg.data(), g, hg.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
             s = RVSP_ERROR_UNSUPPORTED_BACKEND;
         } else {
             memcpy(grad_host, hg.data(), (size_t)np * 4);
             if (loss) *loss = hg[np] / (float)n;
         }
     }
-    (void)hipFree(dp);
-    (void)hipFree(part);
-    (void)hipFree(g);
+    sel_free(dp);
+    sel_free(part);
+    sel_free(g);
     return s;
 }
 
@@ -572,20 +587,20 @@ rvsp_status_t rvsp_selector_train(int32_t which, const float *feats, const float
     float *dp = nullptr, *part = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
     int32_t *didx = nullptr;
     rvsp_status_t s = RVSP_SUCCESS;
-    if (hipMalloc((void **)&dp, (size_t)np * 4) != hipSuccess ||
-        hipMalloc((void **)&part, (size_t)nbmax * (np + 2) * 4) != hipSuccess ||
-        hipMalloc((void **)&g, (size_t)(np + 2) * 4) != hipSuccess ||
-        hipMalloc((void **)&m1, (size_t)np * 4) != hipSuccess ||
-        hipMalloc((void **)&m2, (size_t)np * 4) != hipSuccess ||
-        hipMalloc((void **)&didx, (size_t)n * 4) != hipSuccess)
+    if (sel_malloc((void **)&dp, (size_t)np * 4) != hipSuccess ||
+        sel_malloc((void **)&part, (size_t)nbmax * (np + 2) * 4) != hipSuccess ||
+        sel_malloc((void **)&g, (size_t)(np + 2) * 4) != hipSuccess ||
+        sel_malloc((void **)&m1, (size_t)np * 4) != hipSuccess ||
+        sel_malloc((void **)&m2, (size_t)np * 4) != hipSuccess ||
+        sel_malloc((void **)&didx, (size_t)n * 4) != hipSuccess)
         s = RVSP_ERROR_ALLOCATION_FAILED;
     std::vector<int32_t> perm;
     if (s == RVSP_SUCCESS) {
         perm.resize((size_t)n);
         for (int i = 0; i < n; ++i) perm[i] = i;
-        if (hipMemcpy(dp, params_host, (size_t)np * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemset(m1, 0, (size_t)np * 4) != hipSuccess ||
-            hipMemset(m2, 0, (size_t)np * 4) != hipSuccess)
+        if (sel_copy(dp, params_host, (size_t)np * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            sel_zero(m1, (size_t)np * 4) != hipSuccess ||
+            sel_zero(m2, (size_t)np * 4) != hipSuccess)
             s = RVSP_ERROR_UNSUPPORTED_BACKEND;
     }
     uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
@@ -600,7 +615,7 @@ rvsp_status_t rvsp_selector_train(int32_t which, const float *feats, const float
             perm[i] = perm[j];
             perm[j] = t;
         }
-        if (hipMemcpy(didx, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        if (sel_copy(didx, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess) {
             s = RVSP_ERROR_UNSUPPORTED_BACKEND;
             break;
         }
@@ -612,7 +627,7 @@ rvsp_status_t rvsp_selector_train(int32_t which, const float *feats, const float
             b2t *= 0.999f;
             grad_launch(which, dp, feats, target, didx + off, cnt, part);
             reduce_launch(which, part, nb, g, dp, m1, m2, lr, b1t, b2t, 1);
-            if (hipMemcpy(&lb, g + np, 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            if (sel_copy(&lb, g + np, 4, hipMemcpyDeviceToHost) != hipSuccess) {
                 s = RVSP_ERROR_UNSUPPORTED_BACKEND;
                 break;
             }
@@ -621,15 +636,15 @@ rvsp_status_t rvsp_selector_train(int32_t which, const float *feats, const float
         last = lsum / (double)n;
     }
     if (s == RVSP_SUCCESS &&
-        hipMemcpy(params_host, dp, (size_t)np * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        sel_copy(params_host, dp, (size_t)np * 4, hipMemcpyDeviceToHost) != hipSuccess)
         s = RVSP_ERROR_UNSUPPORTED_BACKEND;
     if (loss_out) *loss_out = (float)last;
-    (void)hipFree(dp);
-    (void)hipFree(part);
-    (void)hipFree(g);
-    (void)hipFree(m1);
-    (void)hipFree(m2);
-    (void)hipFree(didx);
+    sel_free(dp);
+    sel_free(part);
+    sel_free(g);
+    sel_free(m1);
+    sel_free(m2);
+    sel_free(didx);
     return s;
 }
 
