@@ -124,6 +124,56 @@ rvsp_status_t rvsp_spgemm_csr_device_algo(const rvsp_csr_matrix_t *A,
                                           rvsp_csr_matrix_t *C,
                                           rvsp_spgemm_algo_t algo);
 
+/* ---- autotuned launch parameters ------------------------------------------ */
+
+/*
+ * No counterpart in rv-sparse (whose tunables are compile-time macros).
+ * Parameters that decide how a product is binned and scheduled, never what
+ * it computes: every setting gives bit-identical C.
+ *   big_window_outputs/chunks  big-row numeric window: one of 2048x16,
+ *                              4096x32, 2048x32, 1024x16, 1024x8
+ *   sym_big_threads            big-row count block: 256, 512 or 1024
+ *   seg_products               products per big-row segment (one wave),
+ *                              1024 .. 2^22
+ *   hash_max_u                 rows whose min(products, column span) is at
+ *                              most this use the LDS hash count, 64 .. 2048
+ *   fan_min_products/_fill     products (structure fill: outputs) above
+ *                              which a pass runs its row bins on side streams
+ * One entry per workload class (rvsp_tuning_class: estimated products
+ * nnz(A) * nnz(B) / k in four size buckets, times whether B has more than
+ * 16384 columns), written by the autotuner (python -m rvsp.autotune) to
+ * tuning/table.txt beside the library, or RVSP_TUNING_TABLE, and loaded on
+ * first use. Environment overrides (RVSP_BIG_WINDOW, RVSP_SYM_BIG_THREADS)
+ * still take precedence.
+ */
+#define RVSP_TUNING_CLASSES 8
+
+typedef struct
+{
+    int32_t big_window_outputs;
+    int32_t big_window_chunks;
+    int32_t sym_big_threads;
+    int32_t seg_products;
+    int32_t hash_max_u;
+    int64_t fan_min_products;
+    int64_t fan_min_fill;
+} rvsp_tuning_t;
+
+/* Class of a product C = A * B (A m x k, B k x n) from its shapes alone. */
+int32_t rvsp_tuning_class(int64_t nnz_a, int64_t nnz_b, int32_t k, int32_t n);
+/* The built-in defaults (the values before any tuning). */
+rvsp_status_t rvsp_tuning_defaults(rvsp_tuning_t *out);
+rvsp_status_t rvsp_tuning_get(int32_t cls, rvsp_tuning_t *out);
+/* Replaces one class's entry (INVALID_ARGUMENT for values outside the
+ * ranges above). */
+rvsp_status_t rvsp_tuning_set(int32_t cls, const rvsp_tuning_t *t);
+/* Uses t for every class until called with NULL (the autotuner's trials). */
+rvsp_status_t rvsp_tuning_force(const rvsp_tuning_t *t);
+/* Loads a table file (format: '#' comments, then per class the line
+ * "class outputs chunks sym_big_threads seg_products hash_max_u
+ * fan_min_products fan_min_fill"); classes it omits get the defaults. */
+rvsp_status_t rvsp_tuning_load_file(const char *path);
+
 /* ---- allocation-free device one-shot --------------------------------------- */
 
 /*

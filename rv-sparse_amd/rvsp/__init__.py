@@ -66,6 +66,27 @@ class Options(C.Structure):
     _fields_ = [("backend", C.c_int), ("input_dtype", C.c_int), ("output_dtype", C.c_int)]
 
 
+class Tuning(C.Structure):
+    """rvsp_tuning_t (rv_sparse_hip.h)."""
+    _fields_ = [("big_window_outputs", C.c_int32), ("big_window_chunks", C.c_int32),
+                ("sym_big_threads", C.c_int32), ("seg_products", C.c_int32),
+                ("hash_max_u", C.c_int32), ("fan_min_products", C.c_int64),
+                ("fan_min_fill", C.c_int64)]
+
+    FIELDS = ("big_window_outputs", "big_window_chunks", "sym_big_threads", "seg_products",
+              "hash_max_u", "fan_min_products", "fan_min_fill")
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f in self.FIELDS}
+
+    @staticmethod
+    def of(d) -> "Tuning":
+        return Tuning(*[int(d[f]) for f in Tuning.FIELDS])
+
+
+TUNING_CLASSES = 8
+
+
 class GenmatParams(C.Structure):
     """genmat_params_t (include/genmat.h)."""
     _fields_ = [("row_cnt", C.c_int), ("col_cnt", C.c_int), ("density", C.c_double),
@@ -147,6 +168,13 @@ def _declare(lib):
         "rvsp_spgemm_batched_device": (C.c_int, [I32, CSRP, CSRP, CSRP, C.c_int, P(V)]),
         "rvsp_spgemm_batch_chunks": (I32, [V]),
         "rvsp_spgemm_batch_destroy": (None, [V]),
+        # autotuned launch parameters (rv_sparse_hip.h)
+        "rvsp_tuning_class": (I32, [I64, I64, I32, I32]),
+        "rvsp_tuning_defaults": (C.c_int, [P(Tuning)]),
+        "rvsp_tuning_get": (C.c_int, [I32, P(Tuning)]),
+        "rvsp_tuning_set": (C.c_int, [I32, P(Tuning)]),
+        "rvsp_tuning_force": (C.c_int, [P(Tuning)]),
+        "rvsp_tuning_load_file": (C.c_int, [C.c_char_p]),
         # learned selectors (rv_sparse_hip.h)
         "rvsp_selector_param_count": (I32, [I32]),
         "rvsp_selector_set": (C.c_int, [I32, V, I32, C.c_float]),
@@ -519,6 +547,31 @@ def spgemm_device(dA: DeviceCsr, dB: DeviceCsr, algo=ALGO_DEFAULT, keep=False):
         return host, c
     lib().rvsp_csr_destroy(C.byref(c))
     return host
+
+
+def tuning_class(A: "Csr", B: "Csr") -> int:
+    return int(lib().rvsp_tuning_class(A.nnz, B.nnz, A.cols, B.cols))
+
+
+def tuning_defaults() -> dict:
+    t = Tuning()
+    check("tuning_defaults", lib().rvsp_tuning_defaults(C.byref(t)))
+    return t.as_dict()
+
+
+def tuning_get(cls: int) -> dict:
+    t = Tuning()
+    check("tuning_get", lib().rvsp_tuning_get(cls, C.byref(t)))
+    return t.as_dict()
+
+
+def tuning_force(d):
+    """Use the parameters d for every product until tuning_force(None)."""
+    if d is None:
+        check("tuning_force", lib().rvsp_tuning_force(None))
+    else:
+        t = Tuning.of(d)
+        check("tuning_force", lib().rvsp_tuning_force(C.byref(t)))
 
 
 PHASES = ("analyse", "fill", "numeric")

@@ -302,7 +302,7 @@ __global__ __launch_bounds__(NWV * 64) void k_big_fill(int32_t nsegs, BigArgs g,
 // ---------------------------------------------------------------------------
 // Window and segment plan: one thread per big row. Windows are runs of
 // whole chunks with at most g.win_outs outputs and g.win_chunks chunks.
-// Segments are runs of windows holding about kSegProducts products
+// Segments are runs of windows holding about g.seg_products products
 // (estimated from the row's products per output), each processed by one
 // wave, so the widest rows spread over many waves; their cursor blocks
 // (alen entries each) are carved from a running counter.
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
     const int64_t c32 = valid ? g.big_off[row] / kChunkWords : 0;
     const int alen = g.arp[row + 1] - g.arp[row];
     const int nz = crp[row + 1] - crp[row];
-    // products per output of this row: segments aim at ~kSegProducts each
+    // products per output of this row: segments aim at ~g.seg_products each
     const double ratio = nz > 0 ? (double)g.ops[row] / (double)nz : 1.0;
     int nwin = 0;
     if (valid) {
@@ -384,7 +384,7 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
                 const int w = wb + u;
                 if (w >= nwin) break;
                 est += (o8[u + 1] - o8[u]) * ratio;
-                if (est >= (double)kSegProducts || w == nwin - 1) {
+                if (est >= (double)g.seg_products || w == nwin - 1) {
                     if (pass == 1) {
                         g.segs[sbase + s] = make_int4(row, w0, w + 1, 0);
                         g.seg_cur[sbase + s] = cbase + (long long)s * alen;
@@ -721,10 +721,9 @@ static void launch_sym_big_nt(hipStream_t st, const int32_t *rows, int nrows, co
 void launch_sym_big(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g)
 {
     if (nrows <= 0) return;
-    // 512 threads (measured fastest of 256 / 512 / 1024 on R-MAT 2^18 and the
-    // mixed workload); RVSP_SYM_BIG_THREADS picks another for tuning
+    // the tuned block size (g.sym_threads); RVSP_SYM_BIG_THREADS overrides it
     const char *e = getenv("RVSP_SYM_BIG_THREADS");
-    const int nt = e ? atoi(e) : 512;
+    const int nt = e ? atoi(e) : g.sym_threads;
     if (nt == 1024)
         launch_sym_big_nt<1024>(st, rows, nrows, g);
     else if (nt == 256)

@@ -49,13 +49,13 @@ constexpr int kBmsSpillOps = 2 * kBmsWords;  // products above which a bitmap ro
 constexpr int kBigPassWords = 16384;      // big-row bitmap: 2^19 columns per pass
 constexpr int kChunkCols = 1024;          // big-row chunk (window granularity)
 constexpr int kChunkWords = kChunkCols / 32;
-constexpr int kSegProducts = 16384;       // products per big-row segment (one wave)
+constexpr int kSegProducts = 16384;       // default products per big-row segment (one wave)
 constexpr int kDenseMaxCols = 16384;      // widest dense window (64 KB acc)
 constexpr int kHashMaxNnz = 2730;         // numeric hash rows: H = 4096 at load <= 2/3
 constexpr int kRankMaxCols = 16384;       // rank windows: span (512 bitmap words)
 constexpr int kRankWords = kRankMaxCols / 32;
 constexpr int kRankMaxNnz = 4096;         // rank windows: outputs
-constexpr int kSymHashMaxU = 2048;        // symbolic hash rows: min(ops,span) <= this
+constexpr int kSymHashMaxU = 2048;        // largest symbolic hash limit (tables of 4096 keys)
 
 __device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
 

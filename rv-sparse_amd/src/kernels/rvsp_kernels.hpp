@@ -45,8 +45,25 @@ struct BigArgs {
     int win_outs;         // numeric window: at most this many outputs ...
     int win_chunks;       // ... and this many chunks (one of the launchable shapes)
     int32_t nsegs;        // segments planned (set once the analysis has read the count)
+    int seg_products;     // products per segment the plan aims at
+    int sym_threads;      // count block size (256, 512 or 1024)
     int32_t *cols_out;    // numeric pass also writes the big rows' columns here (or null)
 };
+
+// Launch parameters of one analysis, from the tuning table
+// (rvsp_tuning.hip): they change how rows are binned and scheduled, never
+// the result.
+struct TuneParams {
+    int big_window;           // index into kBigShapes
+    int sym_big_threads;      // big-row count block: 256, 512 or 1024
+    int seg_products;         // products per big-row segment
+    int hash_max_u;           // symbolic hash rows: min(ops, span) <= this
+    int64_t fan_min_products; // row bins on side streams from this many products
+    int64_t fan_min_fill;     // ... and for the structure fill, outputs
+};
+// may_load = false: no file is read (the table if already loaded, else the
+// defaults), for callers that must not allocate
+TuneParams tune_for(int64_t nnz_a, int64_t nnz_b, int32_t k, int32_t n, bool may_load = true);
 
 // Canonical-CSR check of a device operand (rvsp_validate.hip): sets
 // kErrOperand in *err on a violation; `starts` holds nnz/32 + 1 words.
@@ -120,7 +137,7 @@ constexpr int64_t kFanMinFill = int64_t(1) << 24;
 
 void launch_row_stats(hipStream_t st, int32_t m, const int32_t *arp, const int32_t *aci,
                       const int32_t *brp, const int32_t *bci, int64_t *ops, int32_t *lo,
-                      int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr);
+                      int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr, int hash_max_u);
 void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int32_t *off,
                         int32_t *cursor, int32_t *perm, int nb);
 // learned: hash bins use the overflow-checking count (a row whose count

@@ -220,11 +220,11 @@ bool bms_spill_off()
     return off == 1;
 }
 
-// Big-row numeric window shape: kBigShapes[0] unless RVSP_BIG_WINDOW names
+// Big-row numeric window shape: the tuned one unless RVSP_BIG_WINDOW names
 // another launchable one as "<outputs>x<chunks>" (read at every analysis).
-int big_window_shape()
+int big_window_shape(int tuned)
 {
-    int shape = 0;
+    int shape = tuned >= 0 && tuned < kBigShapeCount ? tuned : 0;
     const char *e = getenv("RVSP_BIG_WINDOW");
     int o = 0, c = 0;
     if (e && sscanf(e, "%dx%d", &o, &c) == 2)
@@ -324,6 +324,8 @@ struct rvsp_hip_plan {
     char *arena = nullptr;
     size_t arena_cap = 0, arena_used = 0;
     bool single_stream = false;
+
+    TuneParams tp{};  // launch parameters of the current analysis (tuning table)
 };
 
 // Device arrays of a plan: carved from its arena when it has one (released
@@ -345,7 +347,8 @@ static void pfree(rvsp_hip_plan *p, void *ptr)
     if (!p->arena) dfree(ptr, p->stream);
 }
 
-// Whether a pass of `work` units runs its row bins on side streams.
+// Whether a pass of `work` units runs its row bins on side streams (the
+// tuned thresholds: tp.fan_min_products, tp.fan_min_fill).
 static bool fan_out(const rvsp_hip_plan *p, int64_t work, int64_t min_work)
 {
     return !p->single_stream && work >= min_work;
@@ -458,6 +461,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->n = n;
     p->nnz_a = nnz_a;
     p->nnz_b = nnz_b;
+    p->tp = tune_for(nnz_a, nnz_b, k, n, /*may_load=*/p->arena == nullptr);
     p->arp = arp;
     p->aci = aci;
     p->brp = brp;
@@ -538,7 +542,8 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->sized_rows = p->fallback_rows = 0;
 
     // ---- 1. per-row product counts, spans and symbolic bins
-    launch_row_stats(st, m, arp, aci, brp, bci, p->ops, p->lo, p->hi, p->sbin, p->ctr);
+    launch_row_stats(st, m, arp, aci, brp, bci, p->ops, p->lo, p->hi, p->sbin, p->ctr,
+                     p->tp.hash_max_u);
     HIP_TRY(hipGetLastError());
     if (sizer) {
         launch_sel_bins(st, m, sizer, margin, p->ops, arp, p->lo, p->hi, p->sbin, p->hbin, p->ctr);
@@ -584,12 +589,14 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->big.ops = p->ops;
     p->big.cnt = p->cnt;
     p->big.bits_fill = &p->ctr->bits_fill;
+    p->big.seg_products = p->tp.seg_products;
+    p->big.sym_threads = p->tp.sym_big_threads;
     {
         // bitmap pass: the whole span of any row when B has few enough
         // columns (chunk-aligned spans need at most n/1024 + 1 chunks)
         const int64_t need = ((int64_t)n / kChunkCols + 1) * kChunkWords;
         p->big.pass_words = (int)(need < kBigPassWords ? need : kBigPassWords);
-        const int shape = big_window_shape();
+        const int shape = big_window_shape(p->tp.big_window);
         p->big.win_outs = kBigShapes[shape][0];
         p->big.win_chunks = kBigShapes[shape][1];
     }
@@ -637,7 +644,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     HIP_TRY(hipMemsetAsync(p->cnt, 0, M * 4, st));
     const bool sized = sizer && p->hctr.moved > 0;
     launch_sym_count(st, sa, p->sperm, p->sym_count, p->sym_off, sized,
-                     fan_out(p, p->products, kFanMinProducts));
+                     fan_out(p, p->products, p->tp.fan_min_products));
     HIP_TRY(hipGetLastError());
     if (sized) {
         // rows whose learned table overflowed go back to their heuristic
@@ -667,7 +674,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
             // nperm is free until the numeric bins are scattered
             launch_bin_scatter(st, m, p->fbin, p->binoff, p->binoff + 64, p->nperm, SB_COUNT);
             HIP_TRY(hipGetLastError());
-            launch_sym_count(st, sa, p->nperm, fcount, foff, false, fan_out(p, p->products, kFanMinProducts));
+            launch_sym_count(st, sa, p->nperm, fcount, foff, false, fan_out(p, p->products, p->tp.fan_min_products));
             HIP_TRY(hipGetLastError());
             s = scatter_sym();
             if (s != RVSP_SUCCESS) return s;
@@ -736,7 +743,7 @@ rvsp_status_t rvsp_hip_plan_fill(rvsp_hip_plan_t *p, int32_t *cci)
                p->crp, cci,    p->big, p->err, p->ops, p->bms_spill_ok ? p->bms_spill : nullptr};
     PhaseScope timer(p->stream, PH_FILL);
     launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off, false,
-                    fan_out(p, p->nnzc, kFanMinFill));
+                    fan_out(p, p->nnzc, p->tp.fan_min_fill));
     HIP_TRY(hipGetLastError());
     return RVSP_SUCCESS;
 }
@@ -760,10 +767,10 @@ static void queue_numeric(rvsp_hip_plan *p, const float *aval, const float *bval
     const NumArgs a = num_args(p, aval, bval, cci, cval);
     if (p->pm_ready)
         launch_numeric_ranked(p->stream, a, p->nperm, p->num_count, p->num_off, big, p->nseg,
-                              p->prow, p->pmap, fan_out(p, p->products, kFanMinProducts));
+                              p->prow, p->pmap, fan_out(p, p->products, p->tp.fan_min_products));
     else
         launch_numeric(p->stream, a, p->nperm, p->num_count, p->num_off, big, p->nseg,
-                       fan_out(p, p->products, kFanMinProducts));
+                       fan_out(p, p->products, p->tp.fan_min_products));
 }
 
 rvsp_status_t rvsp_hip_plan_numeric(rvsp_hip_plan_t *p, const float *aval, const float *bval,
@@ -787,7 +794,7 @@ rvsp_status_t rvsp_hip_plan_fill_numeric(rvsp_hip_plan_t *p, const float *aval,
                    p->crp, cci,    p->big, p->err, p->ops, p->bms_spill_ok ? p->bms_spill : nullptr};
         PhaseScope timer(p->stream, PH_FILL);
         launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off, /*skip_big=*/true,
-                        fan_out(p, p->nnzc, kFanMinFill));
+                        fan_out(p, p->nnzc, p->tp.fan_min_fill));
         HIP_TRY(hipGetLastError());
     }
     PhaseScope timer(p->stream, PH_NUMERIC);

@@ -19,15 +19,16 @@ namespace rvsp {
 // Per-row statistics: product count, column span of the products, bin.
 // 16 lanes per row; lanes stride over the row's A entries.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int sym_bin_of(int64_t ops, int32_t lo, int32_t hi)
+// hmax (tuned, at most kSymHashMaxU): rows with u <= hmax use a hash table
+__device__ __forceinline__ int sym_bin_of(int64_t ops, int32_t lo, int32_t hi, int hmax)
 {
     if (ops == 0) return SB_EMPTY;
     const int64_t span = (int64_t)hi - lo + 1;
     const int64_t u = ops < span ? ops : span;
     int H = 64;
-    if (u <= kSymHashMaxU) H = max(64, next_pow2((int)(2 * u)));
-    if (span <= 32 * kBmsWords && (u > kSymHashMaxU || span <= 32 * (int64_t)H)) return SB_BMS;
-    if (u <= kSymHashMaxU) return SB_H64 + (ilog2(H) - 6);
+    if (u <= hmax) H = max(64, next_pow2((int)(2 * u)));
+    if (span <= 32 * kBmsWords && (u > hmax || span <= 32 * (int64_t)H)) return SB_BMS;
+    if (u <= hmax) return SB_H64 + (ilog2(H) - 6);
     return SB_BIG;
 }
 
@@ -73,7 +74,8 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
                                                    int32_t *__restrict__ lo_out,
                                                    int32_t *__restrict__ hi_out,
                                                    uint8_t *__restrict__ sbin_out,
-                                                   AnalysisCounters *__restrict__ ctr)
+                                                   AnalysisCounters *__restrict__ ctr,
+                                                   int hash_max_u)
 {
     constexpr int G = 4, U = 8;
     __shared__ int hist[SB_COUNT];
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
         hi = max(hi, __shfl_xor(hi, off, G));
     }
     if (row < m && gl == 0) {
-        const int b = sym_bin_of(ops, lo, hi);
+        const int b = sym_bin_of(ops, lo, hi, hash_max_u);
         ops_out[row] = ops;
         lo_out[row] = ops ? lo : 0;
         hi_out[row] = ops ? hi : -1;
@@ -707,10 +709,11 @@ static void launch_sym_bins(hipStream_t main, const SymArgs &a, const int32_t *p
 
 void launch_row_stats(hipStream_t st, int32_t m, const int32_t *arp, const int32_t *aci,
                       const int32_t *brp, const int32_t *bci, int64_t *ops, int32_t *lo,
-                      int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr)
+                      int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr, int hash_max_u)
 {
     const int grid = (int)(((int64_t)m + kStatsRowsPerBlock - 1) / kStatsRowsPerBlock);
-    k_row_stats<<<grid, 256, 0, st>>>(m, arp, aci, brp, bci, ops, lo, hi, sbin, ctr);
+    const int hmax = hash_max_u < 64 ? 64 : hash_max_u > kSymHashMaxU ? kSymHashMaxU : hash_max_u;
+    k_row_stats<<<grid, 256, 0, st>>>(m, arp, aci, brp, bci, ops, lo, hi, sbin, ctr, hmax);
 }
 
 void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int32_t *off,
