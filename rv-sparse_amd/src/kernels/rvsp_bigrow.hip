@@ -41,6 +41,14 @@ namespace rvsp {
 // the LDS, which costs less than combining them first with lane shuffles
 // (each shuffle is itself an LDS permute instruction).
 // ---------------------------------------------------------------------------
+// Big rows of at most kRegGroups * 64 A entries keep their per-entry cursors
+// in registers in the numeric pass (and need no cursor storage).
+#ifndef RVSP_BIG_REG_GROUPS
+#define RVSP_BIG_REG_GROUPS 2
+#endif
+constexpr int kRegGroups = RVSP_BIG_REG_GROUPS;
+static_assert(kRegGroups >= 1 && kRegGroups <= 4, "register cursors: one round of the search");
+
 constexpr int kSymLong = 48;    // B rows longer than this are shared by a wave
 constexpr int kSymPiece = 512;  // entries per wave piece (8 per lane)
 
@@ -360,7 +368,9 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
         if (pass == 1) {
             int64_t tseg, tcur;
             const int64_t xseg = block_excl_scan(nseg, &tseg);
-            const int64_t xcur = block_excl_scan((int64_t)alen * nseg * (valid ? 1 : 0), &tcur);
+            // rows short enough for register cursors reserve none
+            const int64_t need = alen > kRegGroups * 64 ? (int64_t)alen : 0;
+            const int64_t xcur = block_excl_scan(need * nseg * (valid ? 1 : 0), &tcur);
             if (threadIdx.x == 0) {
                 s_seg = tseg ? atomicAdd(&ctr->nseg, (int)tseg) : 0;
                 s_cur = tcur ? (long long)atomicAdd(&ctr->cur_total, (unsigned long long)tcur) : 0;
@@ -387,7 +397,7 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
                 if (est >= (double)g.seg_products || w == nwin - 1) {
                     if (pass == 1) {
                         g.segs[sbase + s] = make_int4(row, w0, w + 1, 0);
-                        g.seg_cur[sbase + s] = cbase + (long long)s * alen;
+                        g.seg_cur[sbase + s] = cbase + (long long)s * (alen > kRegGroups * 64 ? alen : 0);
                     }
                     ++s;
                     w0 = w + 1;
@@ -447,6 +457,14 @@ __device__ __forceinline__ int big_batch(Acc aj, int32_t col, Acc bv, int32_t c0
 // kPieceBatches batches loaded at once before their adds.
 constexpr int kPieceBatches = 4;
 
+// Entries of a piece taken by its first load (lanes past it load nothing):
+// pieces average about 30 entries, so a narrower first load fetches fewer
+// lines past the window; a piece that fills it continues 64 at a time.
+#ifndef RVSP_BIG_FIRST
+#define RVSP_BIG_FIRST 64
+#endif
+constexpr int kFirstWidth = RVSP_BIG_FIRST;
+
 template <class Acc, class Pair>
 __device__ __forceinline__ int2 big_piece(int cj, int ej, Acc aj, int32_t col, Acc bv, int32_t c0,
                                           int32_t c1, int outs, const uint32_t *W,
@@ -457,7 +475,8 @@ __device__ __forceinline__ int2 big_piece(int cj, int ej, Acc aj, int32_t col, A
     int nin = big_batch(aj, col, bv, c0, c1, outs, W, P, acc, err);
     cj += nin;
     int32_t last = col;  // the batch whose first out-of-window entry ends the piece
-    while (nin == 64) {
+    bool more = nin == kFirstWidth;  // uniform: the batch lay wholly in the window
+    while (more) {
         int32_t cb[kPieceBatches];
         Acc vb[kPieceBatches];
 #pragma unroll
@@ -469,10 +488,11 @@ __device__ __forceinline__ int2 big_piece(int cj, int ej, Acc aj, int32_t col, A
         }
 #pragma unroll
         for (int b = 0; b < kPieceBatches; ++b) {
-            if (nin == 64) {  // uniform: the previous batch lay wholly in the window
+            if (more) {
                 nin = big_batch(aj, cb[b], vb[b], c0, c1, outs, W, P, acc, err);
                 cj += nin;
                 last = cb[b];
+                more = nin == 64;
             }
         }
     }
@@ -487,6 +507,7 @@ __device__ __forceinline__ int2 big_piece(int cj, int ej, Acc aj, int32_t col, A
 #define RVSP_BIG_LIVE 8
 #endif
 constexpr int kLiveGroup = RVSP_BIG_LIVE;
+
 
 // Up to kLiveGroup live A entries of a window, taken from a lane mask in lane
 // (= ascending k) order, with the first 64 entries of each piece loaded
@@ -518,7 +539,7 @@ struct PieceGroup {
             e[u] = lane_bcast(ce.y, j[u]);
             av[u] = lane_bcast(a_val, j[u]);
             const int bp = c[u] + lane;
-            const bool ok = u < n && bp < e[u];
+            const bool ok = u < n && bp < e[u] && lane < kFirstWidth;
             const Pair cv = ok ? pairs[bp] : pair_none<Pair>();
             col[u] = pair_col(cv);
             pair_value(cv, bv[u]);
@@ -557,6 +578,15 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
     const int rowbase = a.crp[row];
     const int nz = a.crp[row + 1] - rowbase;
 
+    constexpr int PER = MAXW / 64;
+    // Rows of at most kRegGroups * 64 A entries keep each entry's state in
+    // registers across the segment's windows (lane l holds entries l,
+    // l + 64, ...): no cursor traffic per window. Longer rows keep it in HBM
+    // (cur_a), read and written back per window.
+    const bool regs = ae - as <= kRegGroups * 64;  // wave-uniform
+    int rcur[kRegGroups], rend[kRegGroups], rcol[kRegGroups];
+    Acc rav[kRegGroups];
+
     // per-entry state at the segment's first column: (cursor, end of B_k,
     // column at the cursor); four binary searches per lane run interleaved
     const int32_t cstart = rlo + g.win_chunk0[c32 + w0] * kChunkCols;
@@ -588,34 +618,58 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
             }
             if (!any) break;
         }
+        if (regs) {  // one round: at most kRegGroups * 64 <= 4 * 64 entries
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int ap = base + u * 64 + lane;
-            const int32_t nc = ap < ae && lo4[u] < e4[u] ? a.bci[lo4[u]] : INT32_MAX;
-            // the group's first cursor also holds the group's smallest column
-            const int gmin = wave_min(nc);
-            if (ap < ae) cur_a[ap - as] = make_int4(lo4[u], e4[u], nc, lane == 0 ? gmin : 0);
+            for (int u = 0; u < kRegGroups; ++u) {
+                const int ap = base + u * 64 + lane;
+                rcur[u] = lo4[u];
+                rend[u] = e4[u];
+                rcol[u] = ap < ae && lo4[u] < e4[u] ? a.bci[lo4[u]] : INT32_MAX;
+                rav[u] = ap < ae ? Acc(a.aval[ap]) : Acc(0);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int ap = base + u * 64 + lane;
+                const int32_t nc = ap < ae && lo4[u] < e4[u] ? a.bci[lo4[u]] : INT32_MAX;
+                // the group's first cursor also holds the group's smallest column
+                const int gmin = wave_min(nc);
+                if (ap < ae) cur_a[ap - as] = make_int4(lo4[u], e4[u], nc, lane == 0 ? gmin : 0);
+            }
         }
     }
 
+    // window w: chunks [wchunk(w), wchunk(w + 1)), outputs [wout(w), wout(w + 1))
+    auto wchunk = [&](int w) { return w < nwin ? g.win_chunk0[c32 + w] : nchunks; };
+    auto wout = [&](int w) { return w < nwin ? g.win_out[c32 + w] : nz; };
+    // up to MAXW bitmap words from chunk ch (lane l: words l * PER ..), past
+    // the row's bitmap zero; words past the window are cleared at use
+    auto load_words = [&](uint32_t *dst, int ch) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int64_t i = (int64_t)ch * kChunkWords + lane * PER + j;
+            dst[j] = i < (int64_t)nchunks * kChunkWords ? g.gbits[boff + i] : 0u;
+        }
+    };
+    int ch0 = wchunk(w0), o0 = wout(w0), ch1 = wchunk(w0 + 1), o1 = wout(w0 + 1);
+    uint32_t wl[PER];
+    load_words(wl, ch0);
+
     for (int w = w0; w < w1; ++w) {
-        const int ch0 = g.win_chunk0[c32 + w];
-        const int ch1 = w + 1 < nwin ? g.win_chunk0[c32 + w + 1] : nchunks;
-        const int o0 = g.win_out[c32 + w];
-        const int o1 = w + 1 < nwin ? g.win_out[c32 + w + 1] : nz;
+        // the next window's bounds and bitmap words load while this one runs
+        const int ch2 = wchunk(w + 2), o2 = wout(w + 2);
+        uint32_t wn[PER];
+        if (w + 1 < w1) load_words(wn, ch1);
         const int outs = o1 - o0;
         const int32_t c0 = rlo + ch0 * kChunkCols;
         const int64_t c1l = (int64_t)rlo + (int64_t)ch1 * kChunkCols;
         const int32_t c1 = c1l > INT32_MAX ? INT32_MAX : (int32_t)c1l;
         const int nw = (ch1 - ch0) * kChunkWords;  // <= MAXW
         // rank dictionary: words and their exclusive popcount prefix
-        constexpr int PER = MAXW / 64;
         int lsum = 0;
-        uint32_t wl[PER];
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const int i = lane * PER + j;
-            wl[j] = i < nw ? g.gbits[boff + (int64_t)ch0 * kChunkWords + i] : 0u;
+            if (lane * PER + j >= nw) wl[j] = 0u;
             lsum += __popc(wl[j]);
         }
         int incl = lsum;
@@ -635,6 +689,31 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
         for (int i = lane; i < outs; i += 64) acc[i] = Acc(0);
         wave_sync();
 
+        if (regs) {
+            // entry groups in A order, live entries of a group in lane order
+#pragma unroll
+            for (int u = 0; u < kRegGroups; ++u) {
+                if (as + u * 64 >= ae) break;  // uniform
+                uint64_t mask = __ballot(rcur[u] < rend[u] && rcol[u] < c1);
+                while (mask) {
+                    PieceGroup<Acc, Pair> g0;
+                    g0.take(mask, make_int4(rcur[u], rend[u], 0, 0), rav[u], pairs);
+#pragma unroll
+                    for (int v = 0; v < kLiveGroup; ++v) {
+                        if (v < g0.n) {
+                            const int2 nc = big_piece(g0.c[v], g0.e[v], g0.av[v], g0.col[v],
+                                                      g0.bv[v], c0, c1, outs, W, P, acc, a.err,
+                                                      pairs);
+                            compiler_fence();
+                            if (lane == g0.j[v]) {
+                                rcur[u] = nc.x;
+                                rcol[u] = nc.y;
+                            }
+                        }
+                    }
+                }
+            }
+        } else {
         // groups of 64 entries whose smallest next column lies before the
         // window's end, found 64 groups at a time; the others are skipped
         for (int gbase = as; gbase < ae; gbase += 64 * 64) {
@@ -677,6 +756,7 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
                 if (live || lane == 0) cur_a[ap - as] = make_int4(cur, ce.y, nxt, lane == 0 ? gmin : 0);
             }
         }
+        }
         wave_sync();
         for (int i = lane; i < outs; i += 64) a.cval[rowbase + o0 + i] = acc[i];
         wave_sync();
@@ -698,6 +778,12 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
             for (int i = lane; i < outs && i < PW; i += 64) g.cols_out[rowbase + o0 + i] = stage[i];
             wave_sync();
         }
+        ch0 = ch1;
+        o0 = o1;
+        ch1 = ch2;
+        o1 = o2;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) wl[j] = wn[j];
     }
 }
 
