@@ -112,7 +112,8 @@ rvsp_status_t rvsp_hip_plan_numeric_graph(rvsp_hip_plan_t *plan,
 
 /* Prepares numeric passes under a reused structure: computes, once per
  * analysis, the output rank of every product of the rows on the LDS paths
- * (2 bytes per product, from C's column indices in c_col_idx) so later
+ * (1 byte per product for rows of at most 256 outputs, else 2, from C's
+ * column indices in c_col_idx) so later
  * numeric passes need no column reads or probing. Skipped (the plain
  * kernels stay in use) when it would take more than a quarter of the free
  * device memory or when RVSP_REUSE_RANKS=0. Queued on the plan stream

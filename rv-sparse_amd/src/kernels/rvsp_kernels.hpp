@@ -198,15 +198,16 @@ void launch_drop_zeros(hipStream_t st, int32_t m, const int32_t *crp, const int3
                        int32_t *rp_out, int32_t *ci_out, int32_t *cv_out);
 
 // reused structure (rvsp_reuse.hip): per-product output ranks of the rows on
-// the LDS paths; with pmap null only the per-row product offsets are made
+// the LDS paths (a byte map: 1- or 2-byte ranks per row); with pmap null only
+// the per-row byte offsets and their total are made
 void launch_pm_build(hipStream_t st, int32_t m, const uint8_t *nbin, const int64_t *ops,
                      const int32_t *arp, const int32_t *aci, const int32_t *brp,
                      const int32_t *bci, const int32_t *crp, const int32_t *cci, int64_t *cnt,
-                     int64_t *tsum, int64_t *total, int64_t *prow, uint16_t *pmap, int *err);
+                     int64_t *tsum, int64_t *total, int64_t *prow, uint8_t *pmap, int *err);
 int pm_scan_tiles(int32_t m);
 void launch_numeric_ranked(hipStream_t st, const NumArgs &a, const int32_t *perm,
                            const int32_t *count, const int32_t *off, const BigArgs &big,
-                           int nsegs, const int64_t *prow, const uint16_t *pmap,
+                           int nsegs, const int64_t *prow, const uint8_t *pmap,
                            bool fan = false);
 
 // learned selectors (rvsp_selector.hip)

@@ -309,9 +309,9 @@ struct rvsp_hip_plan {
 
     // reused structure: per-product output ranks of the LDS-path rows
     void *pmblock = nullptr;  // per-row product offsets (int64), scan scratch
-    uint16_t *pmap = nullptr;
+    uint8_t *pmap = nullptr;  // 1- or 2-byte ranks per row, at prow[row] bytes
     int64_t *prow = nullptr;
-    int64_t pm_products = 0;
+    int64_t pm_bytes = 0;
     bool pm_ready = false, pm_skip = false;
 
     hipGraph_t graph = nullptr;
@@ -375,7 +375,7 @@ static void plan_release(rvsp_hip_plan *p)
     p->pmblock = nullptr;
     p->pmap = nullptr;
     p->prow = nullptr;
-    p->pm_products = 0;
+    p->pm_bytes = 0;
     p->pm_ready = p->pm_skip = false;
     pfree(p, p->rowblock);
     pfree(p, p->bigblock);
@@ -902,12 +902,12 @@ rvsp_status_t rvsp_hip_plan_prepare_reuse(rvsp_hip_plan_t *p, const int32_t *cci
     launch_pm_build(st, p->m, p->nbin, p->ops, p->arp, p->aci, p->brp, p->bci, p->crp, cci, cnt,
                     tsum, tot, p->prow, nullptr, p->err);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(&p->pm_products, tot, 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&p->pm_bytes, tot, 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    // 2 bytes per product; keep to a quarter of the free device memory
+    // 1 or 2 bytes per product; keep to a quarter of the free device memory
     size_t free_b = 0, total_b = 0;
-    const size_t need = (size_t)p->pm_products * 2;
-    if (p->pm_products <= 0 || hipMemGetInfo(&free_b, &total_b) != hipSuccess || need > free_b / 4 ||
+    const size_t need = (size_t)p->pm_bytes;
+    if (p->pm_bytes <= 0 || hipMemGetInfo(&free_b, &total_b) != hipSuccess || need > free_b / 4 ||
         dalloc((void **)&p->pmap, need, st) != RVSP_SUCCESS) {
         dfree(p->pmblock, st);
         p->pmblock = nullptr;

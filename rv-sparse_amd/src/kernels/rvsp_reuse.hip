@@ -6,7 +6,10 @@
 // columns. For rows on the LDS paths (hash and dense bins: at most 16384
 // outputs, so a rank fits 16 bits) the plan computes that rank once per
 // product, in the row's walk order (A entries in stored order, each B row in
-// order), and keeps it in HBM: 2 bytes per product. A numeric pass then adds
+// order), and keeps it in HBM: 1 byte per product for rows of at most 256
+// outputs, 2 bytes otherwise (each row's map starts at an even byte offset).
+// The rank stream is the pass's only traffic without reuse, so the byte
+// ranks cut what a pass reads by up to a third. A numeric pass then adds
 // fl(a*b) into acc[rank] -- a dense LDS accumulator over the row's OUTPUT
 // positions -- with no column read and no probing, in the same ascending-k
 // order as every other kernel here (one group of lanes per row, one LDS add
@@ -20,13 +23,24 @@ namespace rvsp {
 
 __device__ __forceinline__ bool small_bin(int b) { return b >= NB_HASH0 && b < NB_WIN; }
 
-// Products of each row on the LDS paths (0 for empty and big rows).
+// Rows of at most this many outputs keep 1-byte ranks.
+constexpr int kByteRankMax = 256;
+
+// Bytes of each row's rank map (0 for empty and big rows): its products at
+// 1 byte (rounded up to even, so 2-byte maps stay aligned) or 2 bytes each.
 __global__ __launch_bounds__(256) void k_pm_count(int32_t m, const int64_t *__restrict__ ops,
                                                   const uint8_t *__restrict__ nbin,
+                                                  const int32_t *__restrict__ crp,
                                                   int64_t *__restrict__ cnt)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < m) cnt[i] = small_bin(nbin[i]) ? ops[i] : 0;
+    if (i >= m) return;
+    if (!small_bin(nbin[i])) {
+        cnt[i] = 0;
+        return;
+    }
+    const bool narrow = crp[i + 1] - crp[i] <= kByteRankMax;
+    cnt[i] = narrow ? (ops[i] + 1) & ~(int64_t)1 : 2 * ops[i];
 }
 
 // Exclusive int64 scan of cnt into off (three launches, tiles of 2048).
@@ -94,7 +108,7 @@ __global__ __launch_bounds__(256) void k_pm_build(int32_t m, const uint8_t *__re
                                                   const int32_t *__restrict__ crp,
                                                   const int32_t *__restrict__ cci,
                                                   const int64_t *__restrict__ prow,
-                                                  uint16_t *__restrict__ pmap, int *err)
+                                                  uint8_t *__restrict__ pmap, int *err)
 {
     __shared__ int32_t stage[4][kPmStage];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -110,7 +124,10 @@ __global__ __launch_bounds__(256) void k_pm_build(int32_t m, const uint8_t *__re
     if (staged)
         for (int q = lane; q < nz; q += 64) stage[wv][q] = cci[cs + q];
     wave_sync();
-    int64_t p = prow[row];
+    const bool narrow = nz <= kByteRankMax;
+    uint8_t *m8 = pmap + prow[row];
+    uint16_t *m16 = reinterpret_cast<uint16_t *>(m8);  // even offset for 2-byte maps
+    int64_t p = 0;
     const int as = arp[row], ae = arp[row + 1];
     for (int ap = as; ap < ae; ++ap) {
         const int k = aci[ap];
@@ -126,7 +143,11 @@ __global__ __launch_bounds__(256) void k_pm_build(int32_t m, const uint8_t *__re
                     hi = mid;
             }
             if (lo >= nz || cols[lo] != col) atomicOr(err, kErrProbeMiss);
-            pmap[p + (bp - bs)] = (uint16_t)(lo < nz ? lo : 0);
+            const int r = lo < nz ? lo : 0;
+            if (narrow)
+                m8[p + (bp - bs)] = (uint8_t)r;
+            else
+                m16[p + (bp - bs)] = (uint16_t)r;
         }
         p += be - bs;
     }
@@ -137,12 +158,14 @@ __global__ __launch_bounds__(256) void k_pm_build(int32_t m, const uint8_t *__re
 // k-steps are issued before their adds. A k-step's entry reaches the lanes
 // through a scalar register (G = 64) or the group's LDS table M.
 template <int G, typename Add>
-__device__ __forceinline__ void walk_row_ranked(const NumArgs &a, int as, int ae, int64_t pbase,
-                                                const uint16_t *__restrict__ pmap, KMeta *M,
-                                                Add add)
+__device__ __forceinline__ void walk_row_ranked(const NumArgs &a, int as, int ae,
+                                                const uint8_t *__restrict__ rmap, bool narrow,
+                                                KMeta *M, Add add)
 {
     const int lane = threadIdx.x % G;
-    int64_t run = pbase;  // first product of the current chunk of G A entries
+    const uint16_t *__restrict__ rmap16 = reinterpret_cast<const uint16_t *>(rmap);
+    auto rank = [&](int64_t q) { return narrow ? (int)rmap[q] : (int)rmap16[q]; };
+    int64_t run = 0;  // first product (row-relative) of the current chunk of G A entries
     for (int base = as; base < ae; base += G) {
         const int ap = base + lane;
         float av = 0.0f;
@@ -187,7 +210,7 @@ __device__ __forceinline__ void walk_row_ranked(const NumArgs &a, int as, int ae
                 }
                 const int bp = su[u] + lane;
                 const bool ok = j0 + u < nch && bp < eu[u];
-                ru[u] = ok ? (int)pmap[run + pu[u] + lane] : -1;
+                ru[u] = ok ? rank(run + pu[u] + lane) : -1;
                 vu[u] = ok ? a.bval[bp] : 0.0f;
             }
 #pragma unroll
@@ -195,7 +218,7 @@ __device__ __forceinline__ void walk_row_ranked(const NumArgs &a, int as, int ae
                 if (j0 + u < nch) {  // uniform within the group
                     if (ru[u] >= 0) add(ru[u], __fmul_rn(au[u], vu[u]));
                     for (int bp = su[u] + G + lane; bp < eu[u]; bp += G)
-                        add((int)pmap[run + pu[u] + (bp - su[u])], __fmul_rn(au[u], a.bval[bp]));
+                        add(rank(run + pu[u] + (bp - su[u])), __fmul_rn(au[u], a.bval[bp]));
                     compiler_fence();
                 }
             }
@@ -216,7 +239,7 @@ template <int G, int CAP, int R>
 __global__ __launch_bounds__(R *G) void k_num_ranked(const int32_t *__restrict__ rows,
                                                      int32_t nrows, NumArgs a,
                                                      const int64_t *__restrict__ prow,
-                                                     const uint16_t *__restrict__ pmap)
+                                                     const uint8_t *__restrict__ pmap)
 {
     __shared__ float accs[R * CAP];
     __shared__ KMeta meta[G < 64 ? R * G : 1];
@@ -230,8 +253,9 @@ __global__ __launch_bounds__(R *G) void k_num_ranked(const int32_t *__restrict__
     for (int i = lane; i < nz; i += G) S[i] = 0.0f;
     __syncthreads();
     if (valid) {
-        walk_row_ranked<G>(a, a.arp[row], a.arp[row + 1], prow[row], pmap,
-                           meta + (G < 64 ? g * G : 0), [&](int pos, float prod) {
+        walk_row_ranked<G>(a, a.arp[row], a.arp[row + 1], pmap + prow[row],
+                           a.crp[row + 1] - cs <= kByteRankMax, meta + (G < 64 ? g * G : 0),
+                           [&](int pos, float prod) {
                                if (pos < nz) atomicAdd(&S[pos], prod);
                            });
     }
@@ -241,7 +265,7 @@ __global__ __launch_bounds__(R *G) void k_num_ranked(const int32_t *__restrict__
 
 template <int G, int CAP>
 static void launch_ranked(hipStream_t st, const NumArgs &a, const int32_t *rows, int nrows,
-                          const int64_t *prow, const uint16_t *pmap)
+                          const int64_t *prow, const uint8_t *pmap)
 {
     constexpr int R = (256 / G) < (16384 / CAP) ? (256 / G) : (16384 / CAP);
     static_assert(R * G >= 64, "block must hold at least one full wave");
@@ -265,7 +289,7 @@ static void launch_ranked(hipStream_t st, const NumArgs &a, const int32_t *rows,
         return true;
 
 static bool launch_ranked_bin(hipStream_t st, const NumArgs &a, int bin, const int32_t *rows,
-                              int nr, const int64_t *prow, const uint16_t *pmap)
+                              int nr, const int64_t *prow, const uint8_t *pmap)
 {
     switch (bin) {
         RVSP_RANK_HASH(0, 0, 64, 8)
@@ -327,11 +351,11 @@ static bool launch_ranked_bin(hipStream_t st, const NumArgs &a, int bin, const i
 void launch_pm_build(hipStream_t st, int32_t m, const uint8_t *nbin, const int64_t *ops,
                      const int32_t *arp, const int32_t *aci, const int32_t *brp,
                      const int32_t *bci, const int32_t *crp, const int32_t *cci, int64_t *cnt,
-                     int64_t *tsum, int64_t *total, int64_t *prow, uint16_t *pmap, int *err)
+                     int64_t *tsum, int64_t *total, int64_t *prow, uint8_t *pmap, int *err)
 {
     const int grid = (int)(((int64_t)m + 255) / 256);
     const int ntiles = (int)(((int64_t)m + kPmTile - 1) / kPmTile);
-    k_pm_count<<<grid, 256, 0, st>>>(m, ops, nbin, cnt);
+    k_pm_count<<<grid, 256, 0, st>>>(m, ops, nbin, crp, cnt);
     k_pm_scan_tiles<<<ntiles, 256, 0, st>>>(m, cnt, tsum);
     k_pm_scan_top<<<1, 256, 0, st>>>(ntiles, tsum, total);
     k_pm_scan_apply<<<ntiles, 256, 0, st>>>(m, cnt, tsum, prow);
@@ -344,7 +368,7 @@ int pm_scan_tiles(int32_t m) { return (int)(((int64_t)m + kPmTile - 1) / kPmTile
 
 void launch_numeric_ranked(hipStream_t st, const NumArgs &a, const int32_t *perm,
                            const int32_t *count, const int32_t *off, const BigArgs &big,
-                           int nsegs, const int64_t *prow, const uint16_t *pmap, bool fan)
+                           int nsegs, const int64_t *prow, const uint8_t *pmap, bool fan)
 {
     // bins touch disjoint rows: big rows first on the caller's stream, the
     // others over the side streams (none under graph capture)
