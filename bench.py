@@ -417,6 +417,8 @@ def main():
                     help="also time heuristic vs learned vs exhaustive-best numeric bins")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="rmat: skip the stencil-reuse and batched comparisons")
     ap.add_argument("--csv", default=None,
                     help="instead of the JSON bench line: write rv-sparse bench CSV rows (one per "
                          "timed run) for hip_f32, hip_f32_dev and scalar_f32 to this file")
@@ -552,6 +554,26 @@ def main():
     if other is not None:
         out["plain"] = other
         out["learned_vs_plain"] = round(other["ms_per_step"] / ms_step, 4)
+    if args.workload == "rmat" and not args.no_extra:
+        # the other north-star comparisons, measured in the same run:
+        # hipGraph replay vs per-call passes, one batch vs separate calls
+        sub = argparse.Namespace(**vars(args))
+        sub.no_cpu, sub.steps, sub.warmup = True, 20, 3
+        log("other configs: stencil 64^3 reuse ...")
+        r = measure_reuse(sub)
+        sub.steps, sub.warmup = 5, 1
+        log("other configs: batched 512 ...")
+        b = measure_batched(sub)
+        out["other_configs"] = {
+            "stencil_reuse": {k: r["config"][k] for k in
+                              ("graph_ms_per_pass", "per_call_ms_per_pass", "graph_speedup")}
+            | {"gflops_graph": r["value"], "numeric_roofline_frac": r["roofline"]["frac"],
+               "parity": r.get("parity")},
+            "batched_512": {k: b["config"][k] for k in
+                            ("batched_ms", "separate_calls_ms", "batched_speedup", "chunks")}
+            | {"gflops_batched": b["value"], "numeric_roofline_frac": b["roofline"]["frac"],
+               "parity": b.get("parity")},
+        }
     if args.selector_eval:
         from rvsp import selector as SEL
         log("selector evaluation (heuristic / learned / exhaustive-best numeric bins) ...")
@@ -587,7 +609,11 @@ def beside_roofline(products, traffic, numeric_ms):
             "measured_GBps": round(traffic / t / 1e9, 2) if traffic and t > 0 else None}
 
 
-def bench_reuse(args, n_sets=16):
+def bench_reuse(args):
+    print(json.dumps(measure_reuse(args)), flush=True)
+
+
+def measure_reuse(args, n_sets=16):
     """27-point stencil 64^3: analysis once, then numeric passes, each after
     rewriting A's values in place with a freshly drawn set (sets drawn up
     front with the project generator and cycled)."""
@@ -658,10 +684,14 @@ def bench_reuse(args, n_sets=16):
     rec["parity"] = out_parity
     rec["beside_roofline"] = beside_roofline(products, rec["roofline"]["traffic"], num_graph)
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline(A, A)
-    print(json.dumps(rec), flush=True)
+    return rec
 
 
 def bench_batched(args):
+    print(json.dumps(measure_batched(args)), flush=True)
+
+
+def measure_batched(args):
     """512 independent uneven products: one stacked launch set versus 512
     separate device one-shots."""
     L = rvsp.lib()
@@ -744,7 +774,7 @@ def bench_batched(args):
     rec["beside_roofline"] = beside_roofline(products, rec["roofline"]["traffic"], num_ms)
     rec["phases_ms_per_batch"] = {k: round(v[0] / max(args.steps, 1), 4) for k, v in ph.items()}
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline_batch(pairs)
-    print(json.dumps(rec), flush=True)
+    return rec
 
 
 if __name__ == "__main__":
