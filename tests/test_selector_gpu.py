@@ -225,3 +225,54 @@ def test_shipped_weights_load_and_stay_exact(name):
     for wl in ("rmat", "mixed"):
         A, B = W.SMALL[wl]()
         learned_vs_oracle(A, B, f"shipped {name} {wl}")
+
+
+# products the shipped chooser never saw: the same families as its training
+# set (rv-sparse's generator, the mixed power-law x banded product) at seeds
+# that neither the training products (rvsp/selector.py) nor the benchmark use
+HELD_OUT = {
+    "gen-4096-d0.002-s7": lambda: rvsp.gen_bench_pair(4096, 4096, 0.002, 7),
+    "gen-2048-d0.005-s8": lambda: rvsp.gen_bench_pair(2048, 2048, 0.005, 8),
+    "mixed-2^17-s90": lambda: W.mixed(m=1 << 17, k=1 << 13, n=1 << 17, seed_a=90, seed_b=91),
+}
+
+
+def numeric_ms(dA, dB, algo, runs=7):
+    """Median numeric-phase time (HIP events) of `runs` device one-shots,
+    after one untimed call."""
+    rvsp.spgemm_device(dA, dB, algo=algo)
+    ts = []
+    for _ in range(runs):
+        rvsp.reset_timing()
+        rvsp.set_timing(True)
+        rvsp.spgemm_device(dA, dB, algo=algo)
+        rvsp.set_timing(False)
+        ms, calls = rvsp.get_timing()["numeric"]
+        assert calls == 1
+        ts.append(ms)
+    return float(np.median(ts))
+
+
+@pytest.mark.parametrize("name", list(HELD_OUT))
+def test_shipped_chooser_gain_on_held_out_products(name):
+    """What the trained chooser gains over the heuristic bins it replaces:
+    the numeric pass of held-out products with the shipped weights against
+    the same pass with heuristic bins. The chooser may not lose (at most
+    2 % slower, for timing noise) and may not change a bit of C."""
+    path = os.path.join(SEL_DIR, "chooser.txt")
+    if not os.path.exists(path):
+        pytest.skip("no chooser weights shipped")
+    rvsp.selector_load(rvsp.SELECTOR_CHOOSER, path)
+    A, B = HELD_OUT[name]()
+    dA = rvsp.DeviceCsr(A)
+    dB = dA if B is A else rvsp.DeviceCsr(B)
+    plain = rvsp.spgemm_device(dA, dB, algo=rvsp.ALGO_HIP)
+    learned = rvsp.spgemm_device(dA, dB, algo=rvsp.ALGO_HIP_LEARNED)
+    np.testing.assert_array_equal(learned.row_ptr, plain.row_ptr)
+    np.testing.assert_array_equal(learned.col_idx, plain.col_idx)
+    np.testing.assert_array_equal(learned.values.view(np.uint32), plain.values.view(np.uint32))
+    t_plain = numeric_ms(dA, dB, rvsp.ALGO_HIP)
+    t_learned = numeric_ms(dA, dB, rvsp.ALGO_HIP_LEARNED)
+    say(f"[gain] {name}: numeric plain {t_plain:.4f} ms, learned {t_learned:.4f} ms, "
+        f"learned speedup {t_plain / t_learned:.3f}x")
+    assert t_learned <= 1.02 * t_plain, f"{name}: learned {t_learned:.4f} ms > plain {t_plain:.4f} ms"
