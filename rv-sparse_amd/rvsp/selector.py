@@ -12,7 +12,8 @@ loop that feeds them:
            table and window sizes the row needs); every feasible numeric
            accumulator is timed on each group with HIP events
            (rvsp_hip_plan_time_bin), and the samples' targets are the
-           measured log2(ns per row), unmeasured ones NaN.
+           measured log2(ns per row), unmeasured ones NaN; the margin is
+           then picked by timing whole numeric passes.
 
 Training products are generated with seeds and sizes distinct from the
 benchmark workloads. `python -m rvsp.selector train --out DIR` writes
@@ -208,10 +209,30 @@ def structure_ms(prods, select, reps=3):
     return total
 
 
+def passes_ms(prods, select, reps=5):
+    """Milliseconds (HIP events) of each product's numeric pass with
+    heuristic or learned numeric bins, best of reps each."""
+    for p in prods:
+        p.plan.analyse(select)
+        p.plan.fill()
+    set_timing(True)
+    out = []
+    for p in prods:
+        best = None
+        for _ in range(reps):
+            reset_timing()
+            p.plan.numeric()
+            ms = get_timing()["numeric"][0]
+            best = ms if best is None else min(best, ms)
+        out.append(best)
+    set_timing(False)
+    return np.array(out)
+
+
 # ---------------------------------------------------------------------------
 # chooser
 # ---------------------------------------------------------------------------
-def buckets(p: Product, min_rows=64):
+def buckets(p: Product, min_rows=4):
     """Groups of non-big rows with products: {key: row array}."""
     sb, nb = p.plan.bins()
     ok = (nb != 0) & (nb != NB_WIN)
@@ -349,9 +370,35 @@ def train(out_dir, epochs=40, chooser_epochs=400, seed=1, log=print):
     tot = group_totals(table, F, gid, pc)
     log(f"chooser: {F.shape[0]} samples, {len(table)} groups, loss {loss:.4f}, timed groups "
         f"(us) {tot} ({time.time() - t0:.1f}s)")
-    write_params(os.path.join(out_dir, "chooser.txt"), pc, 1.0)
+    # margin on whole numeric passes: the chooser overrides the heuristic's
+    # bin only where it predicts a gain of more than the margin. Each
+    # margin is scored by the learned/heuristic time ratio of every
+    # training product's pass: the smallest geometric mean among margins
+    # whose worst product is at most 2 % slower (else the margin whose
+    # worst product loses least); shipped only if that mean is below 1.
+    base = passes_ms(prods, SELECT_HEURISTIC)
+    margins, scores = {}, {}
+    for mg in (1.0, 1.1, 1.25, 1.5, 2.0, 3.0):
+        selector_set(SELECTOR_CHOOSER, pc, mg)
+        r = passes_ms(prods, SELECT_LEARNED) / np.maximum(base, 1e-6)
+        margins[mg] = {"geomean": float(np.exp(np.mean(np.log(r)))), "worst": float(r.max()),
+                       "worst_product": prods[int(np.argmax(r))].name}
+        scores[mg] = (0, margins[mg]["geomean"]) if r.max() <= 1.02 else (1, r.max())
+    selector_clear(SELECTOR_CHOOSER)
+    best_mg = min(scores, key=scores.get)
+    ship = margins[best_mg]["geomean"] < 1.0
+    log(f"chooser margins (learned/heuristic numeric time over {len(prods)} products, "
+        f"heuristic total {base.sum():.3f} ms): {margins} -> {best_mg}"
+        f"{'' if ship else ' (not faster than the heuristic: not shipped)'}")
+    path = os.path.join(out_dir, "chooser.txt")
+    if ship:
+        write_params(path, pc, best_mg)
+    elif os.path.exists(path):
+        os.remove(path)
     report["chooser"] = {"samples": int(F.shape[0]), "groups": len(table), "loss": loss,
-                         "timed_groups_us": tot}
+                         "timed_groups_us": tot, "heuristic_ms": [float(v) for v in base],
+                         "margin_ms": {str(k): v for k, v in margins.items()}, "margin": best_mg,
+                         "shipped": bool(ship)}
     with open(os.path.join(out_dir, "train_report.json"), "w") as fh:
         json.dump({"report": report, "groups": table}, fh, indent=1)
     for p in prods:

@@ -16,7 +16,7 @@
 //            (carried from window to window; the first cursor of each group
 //            of 64 also holds the group's smallest next column, so groups
 //            with nothing in the window are skipped unread), and adds each product into
-//            acc[rank(col)] with one LDS float add. One wave, entries in
+//            acc[rank(col)] with one LDS read-modify-write. One wave, entries in
 //            ascending k, columns of one B row distinct: the same ordered,
 //            separately rounded accumulation as rv-sparse's scalar kernel
 //            (gustavson_core_f32.inc:31-63, accum_scalar_f32.c:67-70).
@@ -442,7 +442,7 @@ __device__ __forceinline__ int big_batch(Acc aj, int32_t col, Acc bv, int32_t c0
     const uint32_t word = W[wi];
     const int pos = (int)P[wi] + __popc(word & ((1u << (rel & 31)) - 1u));
     const bool hit = inw && ((word >> (rel & 31)) & 1u) && pos < outs;
-    if (hit) atomicAdd(&acc[pos], vmul(aj, bv));
+    if (hit) lds_add(&acc[pos], vmul(aj, bv));
     bad |= in && !hit;
     return __popcll(__ballot(in));
 }

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace rvsp {
@@ -56,8 +58,31 @@ constexpr int kRankMaxCols = 16384;       // rank windows: span (512 bitmap word
 constexpr int kRankWords = kRankMaxCols / 32;
 constexpr int kRankMaxNnz = 4096;         // rank windows: outputs
 constexpr int kSymHashMaxU = 2048;        // largest symbolic hash limit (tables of 4096 keys)
+// A numeric pass runs on the order of 2048 rows at once (256 CUs, a few
+// LDS-sized rows each): a row with at least 1/kCriticalShare of the pass's
+// products can take longer alone than the rest of the pass together.
+constexpr int64_t kCriticalShare = 2048;
 
 __device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
+
+// Adds x into an LDS accumulator slot that no other lane of the same
+// instruction touches (the lanes of one k-step hold distinct columns), as a
+// plain read-modify-write. A wave's LDS operations take effect in issue
+// order, and the compiler keeps program order between accesses it cannot
+// tell apart (k-steps are also separated by compiler fences), so the next
+// k-step's read of the slot sees this write. The value is the one rounding
+// fl(slot + x) that the LDS atomic add gives (integers wrap). On gfx950 the
+// read-modify-write is much the faster: the 27-point stencil's ranked
+// reuse pass takes 0.50 ms with it and 1.04 ms with ds_add_f32, R-MAT
+// 2^18's big-row numeric pass 19.7 ms against 23.9 ms.
+template <class T>
+__device__ __forceinline__ void lds_add(T *slot, T x)
+{
+    if constexpr (std::is_integral<T>::value)
+        *slot = (T)((std::make_unsigned_t<T>)*slot + (std::make_unsigned_t<T>)x);
+    else
+        *slot = *slot + x;
+}
 
 // A big row's bitmap starts at its lowest product column rounded down to a
 // chunk, so its chunks are aligned to global 1024-column boundaries.

@@ -155,8 +155,8 @@ int scan_tiles(int32_t m);
 // needs to leave the heuristic's accumulator.
 void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_t *ops,
                      const int32_t *arp, const uint8_t *sbin, const int32_t *lo,
-                     const int32_t *hi, const float *chooser, float lg_margin, uint8_t *nbin,
-                     AnalysisCounters *ctr);
+                     const int32_t *hi, const float *chooser, float lg_margin,
+                     int64_t products, uint8_t *nbin, AnalysisCounters *ctr);
 void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
                     const int32_t *off, const BigArgs &big, int nsegs, bool fan = false);
 // Writes B's (column, value) pairs for a numeric pass: bcv[i] = (bci[i], bval[i]).

@@ -9,10 +9,11 @@
 //   * one row (or one column window of a row) belongs to lanes of ONE wave;
 //   * the A entries of the row are walked in stored order, and for each
 //     the lanes cover B_k's entries, whose columns are distinct;
-//   * each contribution is one LDS float add, and a wave's LDS operations
-//     execute in issue order, so successive k-steps reach an accumulator
-//     slot in k order. A compiler fence between k-steps keeps the issue
-//     order equal to program order.
+//   * each contribution is one LDS add (a read-modify-write, lds_add, where
+//     that is faster than the LDS atomic: no lane of a k-step shares a slot
+//     with another), and a wave's LDS operations execute in issue order, so
+//     successive k-steps reach an accumulator slot in k order. A compiler
+//     fence between k-steps keeps the issue order equal to program order.
 // Rows are binned (exact nnz(C_i) from the symbolic pass) to the accumulator
 // with the smallest LDS footprint (or the learned chooser's pick):
 //   hash    nnz(C_i) <= 2730: LDS table of H >= 1.5 nnz slots, pre-loaded
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(R *G) void k_num_hash(const int32_t *__restrict__ r
         walk_row_ordered<G>(a, a.arp[row], a.arp[row + 1], meta + (G < 64 ? g * G : 0),
                             [&](int32_t col, float prod) {
                                 const int s = hash_find<H>(T, col, err);
-                                if (s >= 0) atomicAdd(&V[s], prod);
+                                if (s >= 0) lds_add(&V[s], prod);
                             });
     }
     __syncthreads();
@@ -193,6 +194,11 @@ __global__ __launch_bounds__(R *G) void k_num_dense(const int32_t *__restrict__ 
         walk_row_ordered<G>(a, a.arp[row], a.arp[row + 1], meta + (G < 64 ? g * G : 0),
                             [&](int32_t col, float prod) {
                                 const int32_t c = col - wlo;
+                                // the LDS atomic here: dense rows of rv-sparse's
+                                // generated inputs walk long B rows in many
+                                // batches per k-step, and a read-modify-write
+                                // per batch serialises them (generated 16384^2
+                                // numeric 7.4 -> 8.7 ms)
                                 if ((uint32_t)c < (uint32_t)W)
                                     atomicAdd(&S[c], prod);
                                 else
@@ -272,7 +278,7 @@ __global__ __launch_bounds__(R *G) void k_num_rankw(const int32_t *__restrict__ 
                                     const int pos =
                                         (int)Pr[wi] + __popc(word & ((1u << (c & 31)) - 1u));
                                     if (((word >> (c & 31)) & 1u) && pos < nz)
-                                        atomicAdd(&S[pos], prod);
+                                        lds_add(&S[pos], prod);
                                     else
                                         atomicOr(err, kErrProbeMiss);
                                 } else {

@@ -690,7 +690,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
         HIP_TRY(hipGetLastError());
     }
     launch_num_bins(st, m, p->crp, p->ops, arp, p->sbin, p->lo, p->hi, chooser,
-                    cmargin > 1.0f ? log2f(cmargin) : 0.0f, p->nbin, p->ctr);
+                    cmargin > 1.0f ? log2f(cmargin) : 0.0f, p->products, p->nbin, p->ctr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&p->hctr, p->ctr, sizeof(AnalysisCounters), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));

@@ -256,7 +256,7 @@ __global__ __launch_bounds__(R *G) void k_num_ranked(const int32_t *__restrict__
         walk_row_ranked<G>(a, a.arp[row], a.arp[row + 1], pmap + prow[row],
                            a.crp[row + 1] - cs <= kByteRankMax, meta + (G < 64 ? g * G : 0),
                            [&](int pos, float prod) {
-                               if (pos < nz) atomicAdd(&S[pos], prod);
+                               if (pos < nz) lds_add(&S[pos], prod);
                            });
     }
     __syncthreads();

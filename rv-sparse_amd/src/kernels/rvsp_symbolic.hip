@@ -567,7 +567,8 @@ __device__ __forceinline__ int lanes_for(int64_t ops, int alen, int rows_per_blo
 // a span of at most 16384 columns with its 16-bit popcount prefix, 3 KB,
 // plus 4 bytes per output position, CAP >= nnz). The cheapest feasible one
 // wins.
-__device__ __forceinline__ int num_bin_of(int nz, int64_t ops, int alen, int64_t span)
+__device__ __forceinline__ int num_bin_of(int nz, int64_t ops, int alen, int64_t span,
+                                          bool critical)
 {
     const int H = nz <= kHashMaxNnz ? max(64, next_pow2((3 * nz + 1) / 2)) : 0;
     const int Wd = span <= kDenseMaxCols ? max(512, next_pow2((int)span)) : 0;
@@ -576,14 +577,14 @@ __device__ __forceinline__ int num_bin_of(int nz, int64_t ops, int alen, int64_t
     const int64_t fd = Wd ? 4LL * Wd : INT64_MAX;
     const int64_t fr = C ? kRankWords * 6LL + 4LL * C : INT64_MAX;
     if (fr <= fh && fr <= fd) {
-        const int G = max(32, lanes_for(ops, alen, 4));
+        const int G = critical ? 64 : max(32, lanes_for(ops, alen, 4));
         return NB_RANK0 + 2 * (ilog2(C) - 9) + (G == 64 ? 1 : 0);
     }
     if (fd <= fh) {
-        const int G = lanes_for(ops, alen, 16384 / Wd);
+        const int G = critical ? 64 : lanes_for(ops, alen, 16384 / Wd);
         return NB_DENSE0 + 4 * (ilog2(Wd) - 9) + (ilog2(G) - 3);
     }
-    const int G = lanes_for(ops, alen, 8192 / H);
+    const int G = critical ? 64 : lanes_for(ops, alen, 8192 / H);
     return NB_HASH0 + 4 * (ilog2(H) - 6) + (ilog2(G) - 3);
 }
 
@@ -594,7 +595,7 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
                                                   const int32_t *__restrict__ lo,
                                                   const int32_t *__restrict__ hi,
                                                   const float *__restrict__ chooser,
-                                                  float lg_margin,
+                                                  float lg_margin, int64_t products,
                                                   uint8_t *__restrict__ nbin,
                                                   AnalysisCounters *__restrict__ ctr)
 {
@@ -615,8 +616,12 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
         else {
             const int alen = arp[i + 1] - arp[i];
             const int64_t span = (int64_t)hi[i] - lo[i] + 1;
-            const int hb = num_bin_of(nz, ops[i], alen, span);
-            b = chooser ? chooser_pick(P, ops[i], alen, span, nz, hb, lg_margin) : -1;
+            // a row holding at least 1/kCriticalShare of the pass's products
+            // can outlast the rest of the pass: 64 lanes for its latency, and
+            // the chooser (trained on throughput per row) is not asked
+            const bool critical = ops[i] * kCriticalShare >= products;
+            const int hb = num_bin_of(nz, ops[i], alen, span, critical);
+            b = chooser && !critical ? chooser_pick(P, ops[i], alen, span, nz, hb, lg_margin) : -1;
             if (b < 0) b = hb;
         }
         nbin[i] = (uint8_t)b;
@@ -753,12 +758,12 @@ int scan_tiles(int32_t m)
 
 void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_t *ops,
                      const int32_t *arp, const uint8_t *sbin, const int32_t *lo,
-                     const int32_t *hi, const float *chooser, float lg_margin, uint8_t *nbin,
-                     AnalysisCounters *ctr)
+                     const int32_t *hi, const float *chooser, float lg_margin,
+                     int64_t products, uint8_t *nbin, AnalysisCounters *ctr)
 {
     const int grid = (int)(((int64_t)m + 255) / 256);
-    k_num_bins<<<grid, 256, 0, st>>>(m, crp, ops, arp, sbin, lo, hi, chooser, lg_margin, nbin,
-                                     ctr);
+    k_num_bins<<<grid, 256, 0, st>>>(m, crp, ops, arp, sbin, lo, hi, chooser, lg_margin, products,
+                                     nbin, ctr);
 }
 
 } // namespace rvsp

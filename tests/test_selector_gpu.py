@@ -237,20 +237,24 @@ HELD_OUT = {
 }
 
 
-def numeric_ms(dA, dB, algo, runs=7):
-    """Median numeric-phase time (HIP events) of `runs` device one-shots,
-    after one untimed call."""
-    rvsp.spgemm_device(dA, dB, algo=algo)
-    ts = []
-    for _ in range(runs):
-        rvsp.reset_timing()
-        rvsp.set_timing(True)
+def numeric_ms(dA, dB, algos, runs=15):
+    """Median numeric-phase time (HIP events) of `runs` device one-shots
+    per algo, the algos alternating call by call (so clock or cache drift
+    during the measurement affects both alike), after one untimed call
+    each."""
+    for algo in algos:
         rvsp.spgemm_device(dA, dB, algo=algo)
-        rvsp.set_timing(False)
-        ms, calls = rvsp.get_timing()["numeric"]
-        assert calls == 1
-        ts.append(ms)
-    return float(np.median(ts))
+    ts = {algo: [] for algo in algos}
+    for _ in range(runs):
+        for algo in algos:
+            rvsp.reset_timing()
+            rvsp.set_timing(True)
+            rvsp.spgemm_device(dA, dB, algo=algo)
+            rvsp.set_timing(False)
+            ms, calls = rvsp.get_timing()["numeric"]
+            assert calls == 1
+            ts[algo].append(ms)
+    return [float(np.median(ts[algo])) for algo in algos]
 
 
 @pytest.mark.parametrize("name", list(HELD_OUT))
@@ -271,8 +275,7 @@ def test_shipped_chooser_gain_on_held_out_products(name):
     np.testing.assert_array_equal(learned.row_ptr, plain.row_ptr)
     np.testing.assert_array_equal(learned.col_idx, plain.col_idx)
     np.testing.assert_array_equal(learned.values.view(np.uint32), plain.values.view(np.uint32))
-    t_plain = numeric_ms(dA, dB, rvsp.ALGO_HIP)
-    t_learned = numeric_ms(dA, dB, rvsp.ALGO_HIP_LEARNED)
+    t_plain, t_learned = numeric_ms(dA, dB, [rvsp.ALGO_HIP, rvsp.ALGO_HIP_LEARNED])
     say(f"[gain] {name}: numeric plain {t_plain:.4f} ms, learned {t_learned:.4f} ms, "
         f"learned speedup {t_plain / t_learned:.3f}x")
     assert t_learned <= 1.02 * t_plain, f"{name}: learned {t_learned:.4f} ms > plain {t_plain:.4f} ms"
