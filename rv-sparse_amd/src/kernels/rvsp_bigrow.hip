@@ -759,10 +759,14 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
         for (int i = lane; i < outs; i += 64) a.cval[rowbase + o0 + i] = acc[i];
         wave_sync();
         if (COLS) {
+            // round j expands words j * 64 + lane, so a dense run of words
+            // (the low columns of a skewed row) is shared by all lanes
+            // instead of falling to the few lanes that hold it in the
+            // dictionary's layout
             int32_t *stage = reinterpret_cast<int32_t *>(acc);
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
-                const int i = lane * PER + j;
+                const int i = j * 64 + lane;
                 uint32_t x = W[i];  // zero past the window's words
                 int pos = P[i];
                 while (x) {
