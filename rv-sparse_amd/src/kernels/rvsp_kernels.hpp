@@ -135,9 +135,11 @@ constexpr int64_t kFanMinProducts = int64_t(1) << 21;
 // The structure fill writes nnz(C) columns; below this it runs on one stream.
 constexpr int64_t kFanMinFill = int64_t(1) << 24;
 
+// bst: k records of scratch, the B rows' (length, first column, last column)
 void launch_row_stats(hipStream_t st, int32_t m, const int32_t *arp, const int32_t *aci,
-                      const int32_t *brp, const int32_t *bci, int64_t *ops, int32_t *lo,
-                      int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr, int hash_max_u);
+                      int32_t k, const int32_t *brp, const int32_t *bci, int4 *bst, int64_t *ops,
+                      int32_t *lo, int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr,
+                      int hash_max_u);
 void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int32_t *off,
                         int32_t *cursor, int32_t *perm, int nb);
 // learned: hash bins use the overflow-checking count (a row whose count

@@ -487,6 +487,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     const size_t o_binoff = off; off = align256(off + 128 * 4);
     const size_t o_border = off; off = align256(off + kBigOrderWork * 4);
     const size_t o_err = off; off = align256(off + 4);
+    const size_t o_bst = off; off = align256(off + (size_t)k * sizeof(int4));
     rvsp_status_t s = palloc(p, &p->rowblock, off);
     if (s != RVSP_SUCCESS) return s;
     char *base = (char *)p->rowblock;
@@ -506,6 +507,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->binoff = (int32_t *)(base + o_binoff);
     int32_t *border = (int32_t *)(base + o_border);
     p->err = (int *)(base + o_err);
+    int4 *bst = (int4 *)(base + o_bst);
 
     PhaseScope timer(st, PH_ANALYSE);
     HIP_TRY(hipMemsetAsync(p->ctr, 0, sizeof(AnalysisCounters), st));
@@ -542,7 +544,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->sized_rows = p->fallback_rows = 0;
 
     // ---- 1. per-row product counts, spans and symbolic bins
-    launch_row_stats(st, m, arp, aci, brp, bci, p->ops, p->lo, p->hi, p->sbin, p->ctr,
+    launch_row_stats(st, m, arp, aci, k, brp, bci, bst, p->ops, p->lo, p->hi, p->sbin, p->ctr,
                      p->tp.hash_max_u);
     HIP_TRY(hipGetLastError());
     if (sizer) {

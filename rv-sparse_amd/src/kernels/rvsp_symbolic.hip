@@ -32,30 +32,38 @@ __device__ __forceinline__ int sym_bin_of(int64_t ops, int32_t lo, int32_t hi, i
     return SB_BIG;
 }
 
+// Per B row: its length, first column and last column (INT32_MAX and -1
+// when empty), so a row walk reads one 16-byte record per A entry instead
+// of the row bounds and then both end columns.
+__global__ __launch_bounds__(256) void k_brow_stats(int32_t k, const int32_t *__restrict__ brp,
+                                                    const int32_t *__restrict__ bci,
+                                                    int4 *__restrict__ bst)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= k) return;
+    const int bs = brp[r], be = brp[r + 1];
+    bst[r] = make_int4(be - bs, be > bs ? bci[bs] : INT32_MAX, be > bs ? bci[be - 1] : -1, 0);
+}
+
 // Sums |B_k|, and takes the smallest first and largest last column of the
 // B_k, over the A entries ap0 + t * step (t < U, ap0 + t * step < end): the
-// three dependent loads (A column, B row bounds, B end columns) are each
-// issued for all U entries before any is used.
+// two dependent loads (A column, B row record) are each issued for all U
+// entries before any is used.
 template <int U>
 __device__ __forceinline__ void row_stats_step(const int32_t *__restrict__ aci,
-                                               const int32_t *__restrict__ brp,
-                                               const int32_t *__restrict__ bci, int ap0, int step,
+                                               const int4 *__restrict__ bst, int ap0, int step,
                                                int end, int64_t &ops, int32_t &lo, int32_t &hi)
 {
-    int k[U], bs[U], be[U];
+    int k[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) k[u] = ap0 + u * step < end ? aci[ap0 + u * step] : -1;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        bs[u] = k[u] >= 0 ? brp[k[u]] : 0;
-        be[u] = k[u] >= 0 ? brp[k[u] + 1] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        ops += be[u] - bs[u];
-        if (be[u] > bs[u]) {
-            lo = min(lo, bci[bs[u]]);
-            hi = max(hi, bci[be[u] - 1]);
+        if (k[u] >= 0) {
+            const int4 b = bst[k[u]];
+            ops += b.x;
+            lo = min(lo, b.y);
+            hi = max(hi, b.z);
         }
     }
 }
@@ -68,8 +76,7 @@ constexpr int kStatsRowsPerBlock = 64;
 
 __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__restrict__ arp,
                                                    const int32_t *__restrict__ aci,
-                                                   const int32_t *__restrict__ brp,
-                                                   const int32_t *__restrict__ bci,
+                                                   const int4 *__restrict__ bst,
                                                    int64_t *__restrict__ ops_out,
                                                    int32_t *__restrict__ lo_out,
                                                    int32_t *__restrict__ hi_out,
@@ -93,7 +100,7 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
         ae = arp[row + 1];
     }
     const int e1 = min(ae, as + G * U);
-    row_stats_step<U>(aci, brp, bci, as + gl, G, e1, ops, lo, hi);
+    row_stats_step<U>(aci, bst, as + gl, G, e1, ops, lo, hi);
     uint64_t longs = __ballot(gl == 0 && ae > e1);
     while (longs) {
         const int src = __ffsll((unsigned long long)longs) - 1;
@@ -102,7 +109,7 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
         int64_t o2 = 0;
         int32_t lo2 = INT32_MAX, hi2 = -1;
         for (int ap0 = ls + lane; ap0 < le; ap0 += U * 64)
-            row_stats_step<U>(aci, brp, bci, ap0, 64, le, o2, lo2, hi2);
+            row_stats_step<U>(aci, bst, ap0, 64, le, o2, lo2, hi2);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             o2 += __shfl_xor(o2, off, 64);
@@ -713,12 +720,14 @@ static void launch_sym_bins(hipStream_t main, const SymArgs &a, const int32_t *p
 }
 
 void launch_row_stats(hipStream_t st, int32_t m, const int32_t *arp, const int32_t *aci,
-                      const int32_t *brp, const int32_t *bci, int64_t *ops, int32_t *lo,
-                      int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr, int hash_max_u)
+                      int32_t k, const int32_t *brp, const int32_t *bci, int4 *bst, int64_t *ops,
+                      int32_t *lo, int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr,
+                      int hash_max_u)
 {
+    k_brow_stats<<<(int)(((int64_t)k + 255) / 256), 256, 0, st>>>(k, brp, bci, bst);
     const int grid = (int)(((int64_t)m + kStatsRowsPerBlock - 1) / kStatsRowsPerBlock);
     const int hmax = hash_max_u < 64 ? 64 : hash_max_u > kSymHashMaxU ? kSymHashMaxU : hash_max_u;
-    k_row_stats<<<grid, 256, 0, st>>>(m, arp, aci, brp, bci, ops, lo, hi, sbin, ctr, hmax);
+    k_row_stats<<<grid, 256, 0, st>>>(m, arp, aci, bst, ops, lo, hi, sbin, ctr, hmax);
 }
 
 void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int32_t *off,
