@@ -487,14 +487,17 @@ __device__ __forceinline__ int big_batch(Acc aj, int32_t col, Acc bv, int32_t c0
                                          int outs, const uint32_t *W, const uint16_t *P,
                                          Acc *acc)
 {
-    const bool in = col < c1;
-    const int rel = col - c0;
-    const bool inw = in && rel >= 0;
-    const int wi = inw ? rel >> 5 : 0;
+    // one unsigned compare for c0 <= col < c1 (the sentinel INT32_MAX past
+    // the end of B_k is out); a piece starts at its cursor, at or past c0,
+    // so the batch's in-window entries are a prefix of it
+    const uint32_t rel = (uint32_t)(col - c0);
+    const bool inw = rel < (uint32_t)(c1 - c0);
+    const int nin = __popcll(__ballot(inw));  // taken while every lane is on
+    const int wi = inw ? (int)(rel >> 5) : 0;
     const uint32_t word = W[wi];
     const int pos = (int)P[wi] + __popc(word & ((1u << (rel & 31)) - 1u));
     if (inw && pos < outs) lds_add(&acc[pos], vmul(aj, bv));
-    return __popcll(__ballot(in));
+    return nin;
 }
 
 // Adds the products of one A entry's piece in [c0, c1) that start at `cj`
