@@ -33,16 +33,15 @@ namespace rvsp {
 // Symbolic count of big rows: one block per row, an LDS bitmap of
 // g.pass_words words (sized at launch from B's column count, at most 2^19
 // columns) per pass over the row's span. Product order does not matter for
-// the structure, so the A entries of a chunk of NT are split by B-row
-// length: a thread walks a short B row alone (four loads in flight); long
-// ones are cut into pieces of kSymPiece entries shared out over the waves,
-// each lane loading runs of four consecutive columns with 16-byte loads.
-// Lanes of one LDS atomic OR that hit the same word are serialised by the
-// LDS, and the dense column runs of skewed B rows put many lanes on one
-// word, so a lane first ORs together the bits of its own ascending columns
-// that share a word and sets them with one atomic (R-MAT 2^18 analysis
-// 6.2 -> 5.3 ms). Combining across lanes would cost shuffles, each itself an
-// LDS permute instruction.
+// the structure, so B's rows are read as (word, bits) pairs, one per
+// distinct 32-column word of a row (built once per analysis by
+// launch_big_words): each pair is one LDS atomic OR of a whole word, and
+// the pairs of one B row name distinct words, so the lanes of one atomic
+// never share a word. R-MAT 2^18's big rows read 1.53e9 pairs for 2.90e9
+// products (hub rows are dense in their low columns). The A entries of a
+// chunk of NT are split by their B row's pair count: a thread walks a short
+// row alone (four loads in flight); long ones are cut into pieces of
+// kSymPiece pairs shared out over the waves, eight loads per lane.
 // ---------------------------------------------------------------------------
 // Big rows of at most kRegGroups * 64 A entries keep their per-entry cursors
 // in registers in the numeric pass (and need no cursor storage).
@@ -52,8 +51,8 @@ namespace rvsp {
 constexpr int kRegGroups = RVSP_BIG_REG_GROUPS;
 static_assert(kRegGroups >= 1 && kRegGroups <= 4, "register cursors: one round of the search");
 
-constexpr int kSymLong = 48;    // B rows longer than this are shared by a wave
-constexpr int kSymPiece = 512;  // entries per wave piece (8 per lane)
+constexpr int kSymLong = 48;    // B rows of more pairs than this are shared by a wave
+constexpr int kSymPiece = 512;  // pairs per wave piece (8 per lane)
 
 // LDS slot of bitmap word w: its low six bits XORed with a multiplicative
 // hash of the rest. Skewed column distributions (R-MAT's bits are each 0
@@ -91,52 +90,38 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
     int total = 0;
     for (int pw0 = 0; pw0 < nwords; pw0 += passw) {
         const int pwn = min(passw, nwords - pw0);
-        const int32_t base_col = blo + pw0 * 32;
-        const int32_t pcols = pwn * 32;
         for (int w = tid; w < pwn; w += NT) bits[lds_word(w)] = 0u;
         __syncthreads();
+        // pair words are absolute: this pass holds words [wbase, wbase + pwn)
+        const int32_t wbase = (blo >> 5) + pw0;
         for (int cb = as; cb < ae; cb += NT) {
             const int myap = cb + tid;
             int mbs = 0, mbe = 0;
             if (myap < ae) {
                 const int k = g.aci[myap];
-                mbs = g.brp[k];
-                mbe = g.brp[k + 1];
+                mbs = g.bwp[k];
+                mbe = g.bwp[k + 1];
             }
             const bool lng = mbe - mbs > kSymLong;
-            // a long row's pieces start at 16-byte boundaries of the column
-            // array: its first piece begins `head` (0-3) entries early
-            const int head = (int)(((uintptr_t)(g.bci + mbs) >> 2) & 3);
             int64_t npieces;
             const int pfx = (int)block_excl_scan<NWV>(
-                lng ? (mbe - mbs + head + kSymPiece - 1) / kSymPiece : 0, &npieces);
+                lng ? (mbe - mbs + kSymPiece - 1) / kSymPiece : 0, &npieces);
             s_pfx[tid] = pfx;
             s_bs[tid] = mbs;
             s_be[tid] = mbe;
             if (tid == 0) s_pfx[NT] = (int)npieces;
             __syncthreads();
             if (!lng) {  // this thread's own short B row
-                // the row's columns ascend: bits of one word are ORed together
-                // and set with one atomic when the word changes
-                int wcur = -1;
-                uint32_t bcur = 0u;
                 for (int bp = mbs; bp < mbe; bp += 4) {
-                    int32_t c[4];
+                    int2 c[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) c[u] = bp + u < mbe ? g.bci[bp + u] - base_col : -1;
+                    for (int u = 0; u < 4; ++u) c[u] = bp + u < mbe ? g.bwm[bp + u] : make_int2(-1, 0);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        if (c[u] >= 0 && c[u] < pcols) {
-                            const int w = c[u] >> 5;
-                            if (w != wcur) {
-                                if (bcur) atomicOr(&bits[lds_word(wcur)], bcur);
-                                wcur = w;
-                                bcur = 0u;
-                            }
-                            bcur |= 1u << (c[u] & 31);
-                        }
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t rel = (uint32_t)(c[u].x - wbase);  // a sentinel is out
+                        if (rel < (uint32_t)pwn) atomicOr(&bits[lds_word((int)rel)], (uint32_t)c[u].y);
+                    }
                 }
-                if (bcur) atomicOr(&bits[lds_word(wcur)], bcur);
             }
             for (int pc = wv; pc < (int)npieces; pc += NWV) {
                 // entry e holding piece pc: the last e with s_pfx[e] <= pc
@@ -148,45 +133,18 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
                     else
                         hi = mid - 1;
                 }
-                const int rbs = s_bs[lo], rbe = s_be[lo];
-                // a lane loads four consecutive columns at once (16 bytes)
-                const int rhead = (int)(((uintptr_t)(g.bci + rbs) >> 2) & 3);
-                const int ps = rbs - rhead + (pc - s_pfx[lo]) * kSymPiece;
-                const int bs = max(ps, rbs), be = min(ps + kSymPiece, rbe);
-                int32_t c[kSymPiece / 64];
+                const int bs = s_bs[lo] + (pc - s_pfx[lo]) * kSymPiece;
+                const int be = min(bs + kSymPiece, s_be[lo]);
+                int2 c[kSymPiece / 64];
 #pragma unroll
-                for (int r = 0; r < kSymPiece / 256; ++r) {
-                    const int i0 = ps + r * 256 + lane * 4;
-                    if (i0 >= bs && i0 + 4 <= be) {
-                        const int4 v = *reinterpret_cast<const int4 *>(g.bci + i0);
-                        c[4 * r + 0] = v.x - base_col;
-                        c[4 * r + 1] = v.y - base_col;
-                        c[4 * r + 2] = v.z - base_col;
-                        c[4 * r + 3] = v.w - base_col;
-                    } else {  // the row's first or last few entries
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const int i = i0 + u;
-                            c[4 * r + u] = i >= bs && i < be ? g.bci[i] - base_col : -1;
-                        }
-                    }
+                for (int u = 0; u < kSymPiece / 64; ++u) {
+                    const int bp = bs + u * 64 + lane;
+                    c[u] = bp < be ? g.bwm[bp] : make_int2(-1, 0);
                 }
-                // each run of four ascending columns: one atomic per distinct word
 #pragma unroll
-                for (int r = 0; r < kSymPiece / 256; ++r) {
-                    uint32_t run = 0u;
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int32_t x = c[4 * r + u];
-                        const bool ok = x >= 0 && x < pcols;
-                        if (ok) run |= 1u << (x & 31);
-                        const int32_t y = u < 3 ? c[4 * r + u + 1] : -1;
-                        const bool same = u < 3 && y >= 0 && y < pcols && (y >> 5) == (x >> 5);
-                        if (ok && !same) {
-                            atomicOr(&bits[lds_word(x >> 5)], run);
-                            run = 0u;
-                        }
-                    }
+                for (int u = 0; u < kSymPiece / 64; ++u) {
+                    const uint32_t rel = (uint32_t)(c[u].x - wbase);
+                    if (rel < (uint32_t)pwn) atomicOr(&bits[lds_word((int)rel)], (uint32_t)c[u].y);
                 }
             }
             __syncthreads();
@@ -210,6 +168,60 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
         __syncthreads();
     }
     if (tid == 0) g.cnt[row] = total;
+}
+
+// ---------------------------------------------------------------------------
+// B's rows as (word, bits) pairs: an entry starts a pair when its word
+// differs from the previous entry's or it opens its row; a scan of those
+// flags places the pairs, and each pair's first entry ORs its run (at most
+// 32 entries: a row's columns are distinct).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bw_flags(int32_t nnz, const int32_t *__restrict__ bci,
+                                                  int32_t *__restrict__ flag)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < nnz) flag[i] = i == 0 || (bci[i] >> 5) != (bci[i - 1] >> 5);
+}
+
+__global__ __launch_bounds__(256) void k_bw_row_starts(int32_t k, const int32_t *__restrict__ brp,
+                                                       int32_t *__restrict__ flag)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r < k && brp[r + 1] > brp[r]) flag[brp[r]] = 1;
+}
+
+__global__ __launch_bounds__(256) void k_bw_write(int32_t nnz, const int32_t *__restrict__ bci,
+                                                  const int32_t *__restrict__ flag,
+                                                  const int32_t *__restrict__ pidx,
+                                                  int2 *__restrict__ bwm)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nnz || !flag[i]) return;
+    uint32_t bits = 1u << (bci[i] & 31);
+    for (int64_t j = i + 1; j < nnz && j < i + 32 && !flag[j]; ++j) bits |= 1u << (bci[j] & 31);
+    bwm[pidx[i]] = make_int2(bci[i] >> 5, (int)bits);
+}
+
+__global__ __launch_bounds__(256) void k_bw_rows(int32_t k, const int32_t *__restrict__ brp,
+                                                 const int32_t *__restrict__ pidx,
+                                                 int32_t *__restrict__ bwp)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r <= k) bwp[r] = pidx[brp[r]];  // pidx[nnz] is the pair total
+}
+
+void launch_big_words(hipStream_t st, int32_t k, int32_t nnz, const int32_t *brp,
+                      const int32_t *bci, int32_t *flag, int32_t *pidx, int64_t *tsum,
+                      int64_t *total, int2 *bwm, int32_t *bwp)
+{
+    const int ge = (int)(((int64_t)nnz + 255) / 256), gr = (int)(((int64_t)k + 256) / 256);
+    if (nnz > 0) {
+        k_bw_flags<<<ge, 256, 0, st>>>(nnz, bci, flag);
+        k_bw_row_starts<<<gr, 256, 0, st>>>(k, brp, flag);
+    }
+    launch_scan(st, nnz, flag, tsum, total, pidx);
+    if (nnz > 0) k_bw_write<<<ge, 256, 0, st>>>(nnz, bci, flag, pidx, bwm);
+    k_bw_rows<<<gr, 256, 0, st>>>(k, brp, pidx, bwp);
 }
 
 // ---------------------------------------------------------------------------

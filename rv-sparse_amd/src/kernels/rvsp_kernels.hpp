@@ -48,6 +48,11 @@ struct BigArgs {
     int seg_products;     // products per segment the plan aims at
     int sym_threads;      // count block size (256, 512 or 1024)
     int32_t *cols_out;    // numeric pass also writes the big rows' columns here (or null)
+    // B's rows as (bitmap word, bits of the row in that word) pairs, one per
+    // distinct 32-column word of a row, ascending: the big-row count ORs
+    // whole words; bwp is their row pointer (k + 1 entries)
+    const int2 *bwm;
+    const int32_t *bwp;
 };
 
 // Launch parameters of one analysis, from the tuning table
@@ -172,6 +177,12 @@ void launch_sym_big(hipStream_t st, const int32_t *rows, int nrows, const BigArg
 constexpr int kBigOrderWork = 512;
 hipError_t launch_big_order(hipStream_t st, int32_t *rows, int nrows, const int64_t *ops,
                             int32_t *work, int32_t *tmp);
+// B's (word, bits) pairs for the big-row count: flag (nnz words) and pidx
+// (nnz + 1) are scratch, tsum holds scan_tiles(nnz) and total one int64;
+// writes bwm (at most nnz pairs) and bwp (k + 1).
+void launch_big_words(hipStream_t st, int32_t k, int32_t nnz, const int32_t *brp,
+                      const int32_t *bci, int32_t *flag, int32_t *pidx, int64_t *tsum,
+                      int64_t *total, int2 *bwm, int32_t *bwp);
 // Big-row structure fill over every planned segment (g.nsegs).
 void launch_big_fill(hipStream_t st, const BigArgs &g, const int32_t *crp, int32_t *cci);
 void launch_big_plan(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,

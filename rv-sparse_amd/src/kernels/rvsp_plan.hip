@@ -289,6 +289,7 @@ struct rvsp_hip_plan {
     bool operands_checked = false;  // the next analysis skips the device check
 
     void *bigblock = nullptr;  // big-row bitmaps, chunk counts, windows, segments
+    void *wordblock = nullptr; // B's (word, bits) pairs for the big-row count (analysis only)
     int4 *cursors = nullptr;
     BigArgs big{};
     int nseg = 0;
@@ -367,6 +368,8 @@ static void plan_release(rvsp_hip_plan *p)
 {
     pfree(p, p->bcv);
     p->bcv = nullptr;
+    pfree(p, p->wordblock);
+    p->wordblock = nullptr;
     pfree(p, p->bms_spill);
     p->bms_spill = nullptr;
     p->bms_spill_ok = false;
@@ -628,6 +631,27 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
         p->big.win_out = (int32_t *)(bb + o_wout);
         p->big.segs = (int4 *)(bb + o_segs);
         p->big.seg_cur = (int64_t *)(bb + o_scur);
+        // B's rows as (word, bits) pairs for the count (freed once counted)
+        const size_t NB = (size_t)nnz_b;
+        size_t wo = 0;
+        const size_t o_flag = wo; wo = align256(wo + NB * 4);
+        const size_t o_pidx = wo; wo = align256(wo + (NB + 1) * 4);
+        const size_t o_wts = wo; wo = align256(wo + (size_t)scan_tiles(nnz_b) * 8);
+        const size_t o_wtot = wo; wo = align256(wo + 8);
+        const size_t o_bwm = wo; wo = align256(wo + NB * 8);
+        const size_t o_bwp = wo; wo = align256(wo + ((size_t)k + 1) * 4);
+        s = palloc(p, &p->wordblock, wo);
+        if (s != RVSP_SUCCESS) {
+            plan_release(p);
+            return s;
+        }
+        char *wb = (char *)p->wordblock;
+        launch_big_words(st, k, nnz_b, brp, bci, (int32_t *)(wb + o_flag), (int32_t *)(wb + o_pidx),
+                         (int64_t *)(wb + o_wts), (int64_t *)(wb + o_wtot), (int2 *)(wb + o_bwm),
+                         (int32_t *)(wb + o_bwp));
+        HIP_TRY(hipGetLastError());
+        p->big.bwm = (const int2 *)(wb + o_bwm);
+        p->big.bwp = (const int32_t *)(wb + o_bwp);
     }
 
     // ---- 2. exact nnz(C_i)
@@ -682,6 +706,11 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
             if (s != RVSP_SUCCESS) return s;
         }
     }
+    // the counts are done: B's word pairs are no longer read
+    pfree(p, p->wordblock);
+    p->wordblock = nullptr;
+    p->big.bwm = nullptr;
+    p->big.bwp = nullptr;
     // ---- 3. row pointer
     launch_scan(st, m, p->cnt, p->tsum, (int64_t *)&p->ctr->nnz_total, p->crp);
     HIP_TRY(hipGetLastError());
