@@ -400,50 +400,13 @@ def bench_csv(args):
                                             for key, e in summary.items()}}), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rmat",
-                    help="rmat, mixed, uniform, stencil, batched, mtx:PATH (a local .mtx file, C=A*A), or "
-                         "gen:ROWS:COLS:DENSITY:SEED (rv-sparse's generated pair, C=A*B)")
-    ap.add_argument("--algo", default="learned", choices=["hip", "learned"],
-                    help="learned: MLP row selectors (the plain heuristic is timed beside it)")
-    ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="first run one device copy of this many MiB (a known-size dispatch "
-                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--selector-eval", action="store_true",
-                    help="also time heuristic vs learned vs exhaustive-best numeric bins")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--no-extra", action="store_true",
-                    help="rmat: skip the stencil-reuse and batched comparisons")
-    ap.add_argument("--csv", default=None,
-                    help="instead of the JSON bench line: write rv-sparse bench CSV rows (one per "
-                         "timed run) for hip_f32, hip_f32_dev and scalar_f32 to this file")
-    ap.add_argument("--csv-label", default=None,
-                    help="the CSV label column (default: rv-sparse's label for the workload)")
-    ap.add_argument("--csv-cpu-runs", type=int, default=1)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU path is a design note, not built.", file=sys.stderr)
-        sys.exit(2)
-    if not rvsp.gpu_available():
-        raise SystemExit("no HIP device visible")
-    if args.copy_probe_mib > 0:
-        copy_probe(args.copy_probe_mib)
-
-    if args.csv:
-        return bench_csv(args)
-    if args.workload == "stencil":
-        return bench_reuse(args)
-    if args.workload == "batched":
-        return bench_batched(args)
+def measure_oneshot(args, workload):
+    """The device one-shot of one workload: the bench record (value, phases,
+    roofline, parity of seeded rows, CPU baseline unless args.no_cpu, and the
+    plain heuristic beside the learned path when args.algo is learned)."""
     t0 = time.time()
-    A, B, same, desc = load_workload(args.workload)
-    log(f"workload {args.workload}: {desc}; nnzA={A.nnz} nnzB={B.nnz} ({time.time() - t0:.1f} s)")
+    A, B, same, desc = load_workload(workload)
+    log(f"workload {workload}: {desc}; nnzA={A.nnz} nnzB={B.nnz} ({time.time() - t0:.1f} s)")
     dA = rvsp.DeviceCsr(A)
     dB = dA if same else rvsp.DeviceCsr(B)
     algo = ALGOS[args.algo]
@@ -507,7 +470,7 @@ def main():
     achieved = alg_bytes / (num_avg_ms * 1e-3) / 1e9
     step_gbps = alg_bytes / (ms_step * 1e-3) / 1e9
 
-    traffic = measured_traffic(args.workload)
+    traffic = measured_traffic(workload)
 
     cpu = None
     if not args.no_cpu:
@@ -554,6 +517,55 @@ def main():
     if other is not None:
         out["plain"] = other
         out["learned_vs_plain"] = round(other["ms_per_step"] / ms_step, 4)
+    if args.selector_eval:
+        from rvsp import selector as SEL
+        log("selector evaluation (heuristic / learned / exhaustive-best numeric bins) ...")
+        out["selector_eval"] = SEL.evaluate_chooser(workload, A, B, log=log)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rmat",
+                    help="rmat, mixed, uniform, stencil, batched, mtx:PATH (a local .mtx file, C=A*A), or "
+                         "gen:ROWS:COLS:DENSITY:SEED (rv-sparse's generated pair, C=A*B)")
+    ap.add_argument("--algo", default="learned", choices=["hip", "learned"],
+                    help="learned: MLP row selectors (the plain heuristic is timed beside it)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one device copy of this many MiB (a known-size dispatch "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--selector-eval", action="store_true",
+                    help="also time heuristic vs learned vs exhaustive-best numeric bins")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="rmat: skip the stencil-reuse and batched comparisons")
+    ap.add_argument("--csv", default=None,
+                    help="instead of the JSON bench line: write rv-sparse bench CSV rows (one per "
+                         "timed run) for hip_f32, hip_f32_dev and scalar_f32 to this file")
+    ap.add_argument("--csv-label", default=None,
+                    help="the CSV label column (default: rv-sparse's label for the workload)")
+    ap.add_argument("--csv-cpu-runs", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU path is a design note, not built.", file=sys.stderr)
+        sys.exit(2)
+    if not rvsp.gpu_available():
+        raise SystemExit("no HIP device visible")
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+
+    if args.csv:
+        return bench_csv(args)
+    if args.workload == "stencil":
+        return bench_reuse(args)
+    if args.workload == "batched":
+        return bench_batched(args)
+    out = measure_oneshot(args, args.workload)
     if args.workload == "rmat" and not args.no_extra:
         # the other north-star comparisons, measured in the same run:
         # hipGraph replay vs per-call passes, one batch vs separate calls
@@ -574,10 +586,28 @@ def main():
             | {"gflops_batched": b["value"], "numeric_roofline_frac": b["roofline"]["frac"],
                "parity": b.get("parity")},
         }
-    if args.selector_eval:
-        from rvsp import selector as SEL
-        log("selector evaluation (heuristic / learned / exhaustive-best numeric bins) ...")
-        out["selector_eval"] = SEL.evaluate_chooser(args.workload, A, B, log=log)
+        # the mixed-structure product (learned vs plain, and the numeric pass
+        # under heuristic, learned and exhaustive-best bins) and the uniform
+        # 4096 product against rv-sparse's scalar kernel on one host core
+        sub.steps, sub.warmup, sub.algo = 10, 2, "learned"
+        sub.no_cpu, sub.selector_eval = True, True
+        log("other configs: mixed 2^20 x 2^16 ...")
+        mx = measure_oneshot(sub, "mixed")
+        sub.no_cpu, sub.selector_eval = args.no_cpu, False
+        log("other configs: uniform 4096 ...")
+        un = measure_oneshot(sub, "uniform")
+
+        def brief(r):
+            return {"gflops": r["value"], "ms_per_step": r["ms_per_step"],
+                    "phases_ms": r["phases_ms"], "plain_gflops": r["plain"]["value"],
+                    "learned_vs_plain": r["learned_vs_plain"],
+                    "numeric_roofline_frac": r["roofline"]["frac"], "parity": r["parity"]}
+
+        out["other_configs"]["mixed"] = brief(mx) | {"selector_eval": mx.get("selector_eval")}
+        cpu = un.get("cpu_baseline")
+        out["other_configs"]["uniform_4096"] = brief(un) | {
+            "cpu_baseline": cpu,
+            "gpu_over_cpu": round(un["value"] / cpu["value"], 1) if cpu else None}
     print(json.dumps(out), flush=True)
 
 
