@@ -70,7 +70,7 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows, int32_t nrows,
                                                 BigArgs g)
 {
-    extern __shared__ uint32_t bits[];
+    extern __shared__ __align__(16) uint32_t bits[];
     __shared__ long long s_off;
     __shared__ int s_part[NT / 64];
     __shared__ int s_pfx[NT + 1], s_bs[NT], s_be[NT];
@@ -90,7 +90,10 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
     int total = 0;
     for (int pw0 = 0; pw0 < nwords; pw0 += passw) {
         const int pwn = min(passw, nwords - pw0);
-        for (int w = tid; w < pwn; w += NT) bits[lds_word(w)] = 0u;
+        // lds_word permutes within whole 64-word blocks, so the words of
+        // [0, pwn) land in [0, pwn rounded up to 64): zeroed 16 bytes a store
+        for (int i = tid; i < ((pwn + 63) & ~63) / 4; i += NT)
+            reinterpret_cast<uint4 *>(bits)[i] = make_uint4(0u, 0u, 0u, 0u);
         __syncthreads();
         // pair words are absolute: this pass holds words [wbase, wbase + pwn)
         const int32_t wbase = (blo >> 5) + pw0;
@@ -618,7 +621,8 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
     using Acc = typename Args::acc_t;
     using Pair = typename Args::pair_t;
     constexpr int MAXW = MAXCH * kChunkWords;
-    __shared__ Acc acc_s[NW][PW];
+    static_assert(PW * sizeof(Acc) % 16 == 0, "window accumulators are zeroed 16 bytes a store");
+    __shared__ __align__(16) Acc acc_s[NW][PW];
     __shared__ uint32_t W_s[NW][MAXW];
     __shared__ uint16_t P_s[NW][MAXW];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -748,7 +752,10 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
             P[i] = (uint16_t)run;
             run += __popc(wl[j]);
         }
-        for (int i = lane; i < outs; i += 64) acc[i] = Acc(0);
+        // zero the window's outputs 16 bytes a store (PW * sizeof(Acc) is a
+        // multiple of 16, so the rounded-up range stays in the accumulator)
+        for (int i = lane; i < (int)((outs * sizeof(Acc) + 15) / 16); i += 64)
+            reinterpret_cast<uint4 *>(acc)[i] = make_uint4(0u, 0u, 0u, 0u);
         wave_sync();
 
         if (regs) {
