@@ -612,6 +612,26 @@ struct PieceGroup {
     }
 };
 
+// Copies n elements from a 16-byte aligned LDS array to global memory
+// aligned only to the element: whole 16-byte groups one store per lane (the
+// target allows 16-byte global accesses at 4-byte alignment), the last one
+// to three elements one at a time, nothing past n.
+template <class T>
+__device__ __forceinline__ void copy_out(T *__restrict__ dst, const T *src, int n)
+{
+    struct __attribute__((aligned(4))) Q {
+        uint32_t x, y, z, w;
+    };
+    constexpr int E = 16 / sizeof(T);
+    const int lane = threadIdx.x & 63;
+    const int nq = n / E;
+    for (int i = lane; i < nq; i += 64) {
+        const uint4 q = reinterpret_cast<const uint4 *>(src)[i];
+        reinterpret_cast<Q *>(dst)[i] = Q{q.x, q.y, q.z, q.w};
+    }
+    for (int i = nq * E + lane; i < n; i += 64) dst[i] = src[i];
+}
+
 // COLS: the pass also writes each window's column indices (g.cols_out),
 // expanded from the rank dictionary into the accumulator's LDS once the
 // values are out, so a one-shot needs no separate big-row structure fill.
@@ -827,7 +847,7 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
         }
         }
         wave_sync();
-        for (int i = lane; i < outs; i += 64) a.cval[rowbase + o0 + i] = acc[i];
+        copy_out(a.cval + rowbase + o0, acc, outs);
         wave_sync();
         if (COLS) {
             // round j expands words j * 64 + lane, so a dense run of words
@@ -848,7 +868,7 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
                 }
             }
             wave_sync();
-            for (int i = lane; i < outs && i < PW; i += 64) g.cols_out[rowbase + o0 + i] = stage[i];
+            copy_out(g.cols_out + rowbase + o0, stage, min(outs, PW));
             wave_sync();
         }
         ch0 = ch1;
