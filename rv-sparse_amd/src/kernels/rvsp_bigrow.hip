@@ -643,8 +643,9 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
     constexpr int MAXW = MAXCH * kChunkWords;
     static_assert(PW * sizeof(Acc) % 16 == 0, "window accumulators are zeroed 16 bytes a store");
     __shared__ __align__(16) Acc acc_s[NW][PW];
-    __shared__ uint32_t W_s[NW][MAXW];
-    __shared__ uint16_t P_s[NW][MAXW];
+    static_assert(MAXW % 256 == 0, "a lane's dictionary words are whole runs of four");
+    __shared__ __align__(16) uint32_t W_s[NW][MAXW];
+    __shared__ __align__(16) uint16_t P_s[NW][MAXW];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t it = (int64_t)blockIdx.x * NW + wv;
     if (it >= nsegs) return;  // whole wave leaves; no block barriers below
@@ -730,11 +731,20 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
     auto wout = [&](int w) { return w < nwin ? g.win_out[c32 + w] : nz; };
     // up to MAXW bitmap words from chunk ch (lane l: words l * PER ..), past
     // the row's bitmap zero; words past the window are cleared at use
+    // (a row's words start at a multiple of 32 in gbits and PER is a
+    // multiple of 4, so each run of four is 16-byte aligned and wholly in or
+    // out of the row)
     auto load_words = [&](uint32_t *dst, int ch) {
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
+        for (int j = 0; j < PER; j += 4) {
             const int64_t i = (int64_t)ch * kChunkWords + lane * PER + j;
-            dst[j] = i < (int64_t)nchunks * kChunkWords ? g.gbits[boff + i] : 0u;
+            const uint4 q = i < (int64_t)nchunks * kChunkWords
+                                ? *reinterpret_cast<const uint4 *>(g.gbits + boff + i)
+                                : make_uint4(0u, 0u, 0u, 0u);
+            dst[j] = q.x;
+            dst[j + 1] = q.y;
+            dst[j + 2] = q.z;
+            dst[j + 3] = q.w;
         }
     };
     int ch0 = wchunk(w0), o0 = wout(w0), ch1 = wchunk(w0 + 1), o1 = wout(w0 + 1);
@@ -765,12 +775,27 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
             if (lane >= o) incl += y;
         }
         int run = incl - lsum;
+        // the lane's PER words and their prefixes, 16 (or 8) bytes a store
+        uint32_t pp[PER / 2];
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            const int i = lane * PER + j;
-            W[i] = wl[j];
-            P[i] = (uint16_t)run;
+        for (int j = 0; j < PER; j += 2) {
+            const uint32_t p0 = (uint32_t)run;
             run += __popc(wl[j]);
+            pp[j / 2] = p0 | ((uint32_t)run << 16);
+            run += __popc(wl[j + 1]);
+        }
+#pragma unroll
+        for (int j = 0; j < PER; j += 4)
+            *reinterpret_cast<uint4 *>(W + lane * PER + j) = make_uint4(wl[j], wl[j + 1], wl[j + 2], wl[j + 3]);
+        if constexpr (PER % 8 == 0) {
+#pragma unroll
+            for (int j = 0; j < PER / 2; j += 4)
+                *reinterpret_cast<uint4 *>(P + lane * PER + 2 * j) =
+                    make_uint4(pp[j], pp[j + 1], pp[j + 2], pp[j + 3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < PER / 2; j += 2)
+                *reinterpret_cast<uint2 *>(P + lane * PER + 2 * j) = make_uint2(pp[j], pp[j + 1]);
         }
         // zero the window's outputs 16 bytes a store (PW * sizeof(Acc) is a
         // multiple of 16, so the rounded-up range stays in the accumulator)
