@@ -353,12 +353,14 @@ __global__ __launch_bounds__(NWV * 64) void k_big_fill(int32_t nsegs, BigArgs g,
                 if (lane >= o) incl += y;
             }
             int pos = run + incl - cnt;
-            uint32_t x = wl[r];
+            // one bound check per word (never failing for the plan's
+            // windows) instead of one per bit
+            uint32_t x = pos + cnt <= PW ? wl[r] : 0u;
+            const int32_t cw = cb0 + r * 64 * 32;
             while (x) {
                 const int b = __ffs(x) - 1;
                 x &= x - 1;
-                if (pos < PW) stage[pos] = cb0 + r * 64 * 32 + b;
-                ++pos;
+                stage[pos++] = cw + b;
             }
             run += lane_bcast(incl, 63);
         }
@@ -885,11 +887,15 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
                 const int i = j * 64 + lane;
                 uint32_t x = W[i];  // zero past the window's words
                 int pos = P[i];
+                // a word's outputs lie in [0, outs), outs <= PW, for every
+                // window the plan makes; one check per word keeps the writes
+                // in the stage whatever the data
+                if (pos + __popc(x) > PW) x = 0u;
+                const int32_t cw = c0 + i * 32;
                 while (x) {
                     const int b = __ffs(x) - 1;
                     x &= x - 1;
-                    if (pos < PW) stage[pos] = c0 + i * 32 + b;
-                    ++pos;
+                    stage[pos++] = cw + b;
                 }
             }
             wave_sync();
