@@ -346,12 +346,7 @@ __global__ __launch_bounds__(NWV * 64) void k_big_fill(int32_t nsegs, BigArgs g,
 #pragma unroll
         for (int r = 0; r < PER; ++r) {
             const int cnt = __popc(wl[r]);
-            int incl = cnt;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int y = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += y;
-            }
+            const int incl = wave_incl_scan_all(cnt);
             int pos = run + incl - cnt;
             // one bound check per word (never failing for the plan's
             // windows) instead of one per bit
@@ -722,7 +717,7 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
                 const int ap = base + u * 64 + lane;
                 const int32_t nc = ap < ae && lo4[u] < e4[u] ? a.bci[lo4[u]] : INT32_MAX;
                 // the group's first cursor also holds the group's smallest column
-                const int gmin = wave_min(nc);
+                const int gmin = wave_min_all(nc);
                 if (ap < ae) cur_a[ap - as] = make_int4(lo4[u], e4[u], nc, lane == 0 ? gmin : 0);
             }
         }
@@ -770,13 +765,7 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
             if (lane * PER + j >= nw) wl[j] = 0u;
             lsum += __popc(wl[j]);
         }
-        int incl = lsum;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += y;
-        }
-        int run = incl - lsum;
+        int run = wave_incl_scan_all(lsum) - lsum;
         // the lane's PER words and their prefixes, 16 (or 8) bytes a store
         uint32_t pp[PER / 2];
 #pragma unroll
@@ -868,7 +857,7 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
                         }
                     }
                 }
-                const int gmin = wave_min(nxt);  // lanes past the row hold INT32_MAX
+                const int gmin = wave_min_all(nxt);  // lanes past the row hold INT32_MAX
                 if (live || lane == 0) cur_a[ap - as] = make_int4(cur, ce.y, nxt, lane == 0 ? gmin : 0);
             }
         }

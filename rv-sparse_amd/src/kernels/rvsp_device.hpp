@@ -239,6 +239,50 @@ __device__ __forceinline__ int wave_min(int v)
     return v;
 }
 
+// Cross-lane steps on DPP (data-parallel primitives: a VALU operand read
+// from another lane of the same row of 16, or broadcast across rows), for
+// code where every lane of the wave is active. __shfl compiles to an LDS
+// permute (ds_bpermute) and a wait per step instead.
+namespace dpp {
+constexpr int kQuadSwap1 = 0xB1;     // quad_perm [1,0,3,2]
+constexpr int kQuadSwap2 = 0x4E;     // quad_perm [2,3,0,1]
+constexpr int kRowMirror = 0x140;    // lane i <- lane 15 - i of its row
+constexpr int kRowHalfMirror = 0x141;  // lane i <- lane 7 - i of its half row
+constexpr int kRowBcast15 = 0x142;   // lane 15 of each row to the next row
+constexpr int kRowBcast31 = 0x143;   // lane 31 to the rows above it
+constexpr int kRowShr = 0x110;       // + k: lane i <- lane i - k of its row
+
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int read(int old, int v)
+{
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xF, false);
+}
+} // namespace dpp
+
+// Minimum over the 64 lanes (all active), as a wave-uniform value.
+__device__ __forceinline__ int wave_min_all(int v)
+{
+    v = min(v, dpp::read<dpp::kQuadSwap1>(INT32_MAX, v));
+    v = min(v, dpp::read<dpp::kQuadSwap2>(INT32_MAX, v));
+    v = min(v, dpp::read<dpp::kRowHalfMirror>(INT32_MAX, v));
+    v = min(v, dpp::read<dpp::kRowMirror>(INT32_MAX, v));      // every lane: its row's minimum
+    v = min(v, dpp::read<dpp::kRowBcast15, 0xA>(INT32_MAX, v));  // rows 1, 3: two rows'
+    v = min(v, dpp::read<dpp::kRowBcast31, 0xC>(INT32_MAX, v));  // row 3: all four rows'
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Inclusive prefix sum over the 64 lanes (all active).
+__device__ __forceinline__ int wave_incl_scan_all(int v)
+{
+    v += dpp::read<dpp::kRowShr + 1>(0, v);
+    v += dpp::read<dpp::kRowShr + 2>(0, v);
+    v += dpp::read<dpp::kRowShr + 4>(0, v);
+    v += dpp::read<dpp::kRowShr + 8>(0, v);                      // within each row of 16
+    v += dpp::read<dpp::kRowBcast15, 0xA>(0, v);                 // rows 1, 3 add the row before
+    v += dpp::read<dpp::kRowBcast31, 0xC>(0, v);                 // rows 2, 3 add rows 0-1
+    return v;
+}
+
 // Exclusive scan over a block of NWV waves (256 threads by default); *total
 // gets the sum. Every thread of the block must call it.
 template <int NWV = 4>
