@@ -486,29 +486,36 @@ __device__ __forceinline__ int lane_lower_bound(const int32_t *__restrict__ a, i
 // Adds one batch of a piece (lanes hold consecutive entries of B_k, so
 // distinct columns) and returns how many of its entries lie in the window.
 // Branch-free apart from the add itself: every lane reads the dictionary
-// (lanes outside the window at word 0). The window's dictionary was built
-// from the same B columns this pass reads, and B's rows reach the kernels
-// canonical (checked on the host or the device before the analysis; the
-// legacy dtypes' B rows are sorted on the host and repeated columns
-// refused), so an in-window column is always in it; `pos < outs` keeps
-// every access inside the window's accumulator whatever the data. (Testing
-// each product's dictionary bit and reporting misses cost 4 % of R-MAT
-// 2^18's numeric pass.)
-template <class Acc>
+// (lanes outside the window at whatever word their offset masks to). The
+// window's dictionary was built from the same B columns this pass reads,
+// and B's rows reach the kernels canonical (checked on the host or the
+// device before the analysis; the legacy dtypes' B rows are sorted on the
+// host and repeated columns refused), so an in-window column is always in
+// it. (Testing each product's dictionary bit and reporting misses cost 4 %
+// of R-MAT 2^18's numeric pass.) Prefixes are clamped to PW when the
+// dictionary is built and a word adds at most 31, so `pos` stays below
+// PW + kAccPad, inside the window's accumulator, whatever the data.
+constexpr int kAccPad = 32;
+
+template <int PW, int MAXW, class Acc>
 __device__ __forceinline__ int big_batch(Acc aj, int32_t col, Acc bv, int32_t c0, int32_t c1,
-                                         int outs, const uint32_t *W, const uint16_t *P,
-                                         Acc *acc)
+                                         const uint32_t *W, const uint16_t *P, Acc *acc)
 {
+    static_assert((MAXW & (MAXW - 1)) == 0, "word offsets are masked to the dictionary");
     // one unsigned compare for c0 <= col < c1 (the sentinel INT32_MAX past
     // the end of B_k is out); a piece starts at its cursor, at or past c0,
     // so the batch's in-window entries are a prefix of it
     const uint32_t rel = (uint32_t)(col - c0);
     const bool inw = rel < (uint32_t)(c1 - c0);
     const int nin = __popcll(__ballot(inw));  // taken while every lane is on
-    const int wi = inw ? (int)(rel >> 5) : 0;
+    const int wi = (int)((rel >> 5) & (uint32_t)(MAXW - 1));
     const uint32_t word = W[wi];
-    const int pos = (int)P[wi] + __popc(word & ((1u << (rel & 31)) - 1u));
-    if (inw && pos < outs) lds_add(&acc[pos], vmul(aj, bv));
+    // bits of the word below the column: v_bfe_u32 takes the width modulo 32
+    int pos = (int)P[wi] + __popc(__builtin_amdgcn_ubfe(word, 0u, rel));
+    // kept whole (v_bcnt_u32_b32 adds the prefix), so the slot address is
+    // one shift-add instead of the compiler's redistributed shifts
+    asm volatile("" : "+v"(pos));
+    if (inw) lds_add(&acc[pos], vmul(aj, bv));
     return nin;
 }
 
@@ -527,14 +534,14 @@ constexpr int kPieceBatches = 4;
 #endif
 constexpr int kFirstWidth = RVSP_BIG_FIRST;
 
-template <class Acc, class Pair>
+template <int PW, int MAXW, class Acc, class Pair>
 __device__ __forceinline__ int2 big_piece(int cj, int ej, Acc aj, int32_t col, Acc bv, int32_t c0,
-                                          int32_t c1, int outs, const uint32_t *W,
-                                          const uint16_t *P, Acc *acc,
+                                          int32_t c1, const uint32_t *W, const uint16_t *P,
+                                          Acc *acc,
                                           const Pair *__restrict__ pairs)
 {
     const int lane = threadIdx.x & 63;
-    int nin = big_batch(aj, col, bv, c0, c1, outs, W, P, acc);
+    int nin = big_batch<PW, MAXW>(aj, col, bv, c0, c1, W, P, acc);
     cj += nin;
     int32_t last = col;  // the batch whose first out-of-window entry ends the piece
     bool more = nin == kFirstWidth;  // uniform: the batch lay wholly in the window
@@ -551,7 +558,7 @@ __device__ __forceinline__ int2 big_piece(int cj, int ej, Acc aj, int32_t col, A
 #pragma unroll
         for (int b = 0; b < kPieceBatches; ++b) {
             if (more) {
-                nin = big_batch(aj, cb[b], vb[b], c0, c1, outs, W, P, acc);
+                nin = big_batch<PW, MAXW>(aj, cb[b], vb[b], c0, c1, W, P, acc);
                 cj += nin;
                 last = cb[b];
                 more = nin == 64;
@@ -639,7 +646,7 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
     using Pair = typename Args::pair_t;
     constexpr int MAXW = MAXCH * kChunkWords;
     static_assert(PW * sizeof(Acc) % 16 == 0, "window accumulators are zeroed 16 bytes a store");
-    __shared__ __align__(16) Acc acc_s[NW][PW];
+    __shared__ __align__(16) Acc acc_s[NW][PW + kAccPad];
     static_assert(MAXW % 256 == 0, "a lane's dictionary words are whole runs of four");
     __shared__ __align__(16) uint32_t W_s[NW][MAXW];
     __shared__ __align__(16) uint16_t P_s[NW][MAXW];
@@ -758,7 +765,8 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
         const int64_t c1l = (int64_t)rlo + (int64_t)ch1 * kChunkCols;
         const int32_t c1 = c1l > INT32_MAX ? INT32_MAX : (int32_t)c1l;
         const int nw = (ch1 - ch0) * kChunkWords;  // <= MAXW
-        // rank dictionary: words and their exclusive popcount prefix
+        // rank dictionary: words and their exclusive popcount prefix (clamped
+        // to PW, which the window's outputs never exceed)
         int lsum = 0;
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
@@ -770,9 +778,9 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
         uint32_t pp[PER / 2];
 #pragma unroll
         for (int j = 0; j < PER; j += 2) {
-            const uint32_t p0 = (uint32_t)run;
+            const uint32_t p0 = (uint32_t)min(run, PW);
             run += __popc(wl[j]);
-            pp[j / 2] = p0 | ((uint32_t)run << 16);
+            pp[j / 2] = p0 | ((uint32_t)min(run, PW) << 16);
             run += __popc(wl[j + 1]);
         }
 #pragma unroll
@@ -806,9 +814,9 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
 #pragma unroll
                     for (int v = 0; v < kLiveGroup; ++v) {
                         if (v < g0.n) {
-                            const int2 nc = big_piece(g0.c[v], g0.e[v], g0.av[v], g0.col[v],
-                                                      g0.bv[v], c0, c1, outs, W, P, acc,
-                                                      pairs);
+                            const int2 nc = big_piece<PW, MAXW>(g0.c[v], g0.e[v], g0.av[v],
+                                                                g0.col[v], g0.bv[v], c0, c1,
+                                                                W, P, acc, pairs);
                             compiler_fence();
                             if (lane == g0.j[v]) {
                                 rcur[u] = nc.x;
@@ -846,9 +854,9 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
 #pragma unroll
                     for (int u = 0; u < kLiveGroup; ++u) {
                         if (u < g0.n) {
-                            const int2 nc = big_piece(g0.c[u], g0.e[u], g0.av[u], g0.col[u],
-                                                      g0.bv[u], c0, c1, outs, W, P, acc,
-                                                      pairs);
+                            const int2 nc = big_piece<PW, MAXW>(g0.c[u], g0.e[u], g0.av[u],
+                                                                g0.col[u], g0.bv[u], c0, c1,
+                                                                W, P, acc, pairs);
                             compiler_fence();
                             if (lane == g0.j[u]) {
                                 cur = nc.x;
