@@ -486,32 +486,35 @@ __device__ __forceinline__ int lane_lower_bound(const int32_t *__restrict__ a, i
 // Adds one batch of a piece (lanes hold consecutive entries of B_k, so
 // distinct columns) and returns how many of its entries lie in the window.
 // Branch-free apart from the add itself: every lane reads the dictionary
-// (lanes outside the window at whatever word their offset masks to). The
-// window's dictionary was built from the same B columns this pass reads,
-// and B's rows reach the kernels canonical (checked on the host or the
-// device before the analysis; the legacy dtypes' B rows are sorted on the
-// host and repeated columns refused), so an in-window column is always in
-// it. (Testing each product's dictionary bit and reporting misses cost 4 %
-// of R-MAT 2^18's numeric pass.) Prefixes are clamped to PW when the
+// (lanes outside the window at whatever slot their column masks to). A
+// piece starts at its cursor, the first entry of B_k at or past the
+// window's first column c0, so `col < c1` is the whole window test, and the
+// dictionary holds word w of the columns at slot w mod MAXW (c0 is a
+// multiple of 1024 columns and a window spans at most MAXW words, so its
+// words take distinct slots and a column's bit index is its low five bits).
+// The window's dictionary was built from the same B columns this pass
+// reads, and B's rows reach the kernels canonical (checked on the host or
+// the device before the analysis; the legacy dtypes' B rows are sorted on
+// the host and repeated columns refused), so an in-window column is always
+// in it. (Testing each product's dictionary bit and reporting misses cost
+// 4 % of R-MAT 2^18's numeric pass.) Prefixes are clamped to PW when the
 // dictionary is built and a word adds at most 31, so `pos` stays below
 // PW + kAccPad, inside the window's accumulator, whatever the data.
 constexpr int kAccPad = 32;
 
 template <int PW, int MAXW, class Acc>
-__device__ __forceinline__ int big_batch(Acc aj, int32_t col, Acc bv, int32_t c0, int32_t c1,
+__device__ __forceinline__ int big_batch(Acc aj, int32_t col, Acc bv, int32_t c1,
                                          const uint32_t *W, const uint16_t *P, Acc *acc)
 {
-    static_assert((MAXW & (MAXW - 1)) == 0, "word offsets are masked to the dictionary");
-    // one unsigned compare for c0 <= col < c1 (the sentinel INT32_MAX past
-    // the end of B_k is out); a piece starts at its cursor, at or past c0,
-    // so the batch's in-window entries are a prefix of it
-    const uint32_t rel = (uint32_t)(col - c0);
-    const bool inw = rel < (uint32_t)(c1 - c0);
+    static_assert((MAXW & (MAXW - 1)) == 0, "dictionary slots are word offsets mod MAXW");
+    // the sentinel INT32_MAX past the end of B_k is out; the batch's
+    // in-window entries are a prefix of it
+    const bool inw = col < c1;
     const int nin = __popcll(__ballot(inw));  // taken while every lane is on
-    const int wi = (int)((rel >> 5) & (uint32_t)(MAXW - 1));
+    const int wi = (int)(((uint32_t)col >> 5) & (uint32_t)(MAXW - 1));
     const uint32_t word = W[wi];
     // bits of the word below the column: v_bfe_u32 takes the width modulo 32
-    int pos = (int)P[wi] + __popc(__builtin_amdgcn_ubfe(word, 0u, rel));
+    int pos = (int)P[wi] + __popc(__builtin_amdgcn_ubfe(word, 0u, (uint32_t)col));
     // kept whole (v_bcnt_u32_b32 adds the prefix), so the slot address is
     // one shift-add instead of the compiler's redistributed shifts
     asm volatile("" : "+v"(pos));
@@ -535,13 +538,13 @@ constexpr int kPieceBatches = 4;
 constexpr int kFirstWidth = RVSP_BIG_FIRST;
 
 template <int PW, int MAXW, class Acc, class Pair>
-__device__ __forceinline__ int2 big_piece(int cj, int ej, Acc aj, int32_t col, Acc bv, int32_t c0,
+__device__ __forceinline__ int2 big_piece(int cj, int ej, Acc aj, int32_t col, Acc bv,
                                           int32_t c1, const uint32_t *W, const uint16_t *P,
                                           Acc *acc,
                                           const Pair *__restrict__ pairs)
 {
     const int lane = threadIdx.x & 63;
-    int nin = big_batch<PW, MAXW>(aj, col, bv, c0, c1, W, P, acc);
+    int nin = big_batch<PW, MAXW>(aj, col, bv, c1, W, P, acc);
     cj += nin;
     int32_t last = col;  // the batch whose first out-of-window entry ends the piece
     bool more = nin == kFirstWidth;  // uniform: the batch lay wholly in the window
@@ -558,7 +561,7 @@ __device__ __forceinline__ int2 big_piece(int cj, int ej, Acc aj, int32_t col, A
 #pragma unroll
         for (int b = 0; b < kPieceBatches; ++b) {
             if (more) {
-                nin = big_batch<PW, MAXW>(aj, cb[b], vb[b], c0, c1, W, P, acc);
+                nin = big_batch<PW, MAXW>(aj, cb[b], vb[b], c1, W, P, acc);
                 cj += nin;
                 last = cb[b];
                 more = nin == 64;
@@ -783,18 +786,21 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
             pp[j / 2] = p0 | ((uint32_t)min(run, PW) << 16);
             run += __popc(wl[j + 1]);
         }
+        // word i of the window sits at slot (c0 / 32 + i) mod MAXW; c0 / 32 is
+        // a multiple of 32, so a lane's PER words stay one aligned run
+        const int slot0 = ((c0 >> 5) + lane * PER) & (MAXW - 1);
 #pragma unroll
         for (int j = 0; j < PER; j += 4)
-            *reinterpret_cast<uint4 *>(W + lane * PER + j) = make_uint4(wl[j], wl[j + 1], wl[j + 2], wl[j + 3]);
+            *reinterpret_cast<uint4 *>(W + slot0 + j) = make_uint4(wl[j], wl[j + 1], wl[j + 2], wl[j + 3]);
         if constexpr (PER % 8 == 0) {
 #pragma unroll
             for (int j = 0; j < PER / 2; j += 4)
-                *reinterpret_cast<uint4 *>(P + lane * PER + 2 * j) =
+                *reinterpret_cast<uint4 *>(P + slot0 + 2 * j) =
                     make_uint4(pp[j], pp[j + 1], pp[j + 2], pp[j + 3]);
         } else {
 #pragma unroll
             for (int j = 0; j < PER / 2; j += 2)
-                *reinterpret_cast<uint2 *>(P + lane * PER + 2 * j) = make_uint2(pp[j], pp[j + 1]);
+                *reinterpret_cast<uint2 *>(P + slot0 + 2 * j) = make_uint2(pp[j], pp[j + 1]);
         }
         // zero the window's outputs 16 bytes a store (PW * sizeof(Acc) is a
         // multiple of 16, so the rounded-up range stays in the accumulator)
@@ -815,13 +821,10 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
                     for (int v = 0; v < kLiveGroup; ++v) {
                         if (v < g0.n) {
                             const int2 nc = big_piece<PW, MAXW>(g0.c[v], g0.e[v], g0.av[v],
-                                                                g0.col[v], g0.bv[v], c0, c1,
-                                                                W, P, acc, pairs);
+                                                                g0.col[v], g0.bv[v], c1, W,
+                                                                P, acc, pairs);
                             compiler_fence();
-                            if (lane == g0.j[v]) {
-                                rcur[u] = nc.x;
-                                rcol[u] = nc.y;
-                            }
+                            lane_set2(rcur[u], rcol[u], nc.x, nc.y, g0.j[v]);
                         }
                     }
                 }
@@ -855,13 +858,10 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
                     for (int u = 0; u < kLiveGroup; ++u) {
                         if (u < g0.n) {
                             const int2 nc = big_piece<PW, MAXW>(g0.c[u], g0.e[u], g0.av[u],
-                                                                g0.col[u], g0.bv[u], c0, c1,
-                                                                W, P, acc, pairs);
+                                                                g0.col[u], g0.bv[u], c1, W,
+                                                                P, acc, pairs);
                             compiler_fence();
-                            if (lane == g0.j[u]) {
-                                cur = nc.x;
-                                nxt = nc.y;
-                            }
+                            lane_set2(cur, nxt, nc.x, nc.y, g0.j[u]);
                         }
                     }
                 }
@@ -882,8 +882,9 @@ __global__ __launch_bounds__(NW * 64) void k_num_big(int32_t nsegs, Args a, BigA
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
                 const int i = j * 64 + lane;
-                uint32_t x = W[i];  // zero past the window's words
-                int pos = P[i];
+                const int si = ((c0 >> 5) + i) & (MAXW - 1);
+                uint32_t x = W[si];  // zero past the window's words
+                int pos = P[si];
                 // a word's outputs lie in [0, outs), outs <= PW, for every
                 // window the plan makes; one check per word keeps the writes
                 // in the stage whatever the data

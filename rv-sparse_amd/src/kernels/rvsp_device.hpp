@@ -103,6 +103,19 @@ struct __align__(16) KMeta {
 // Value of lane `src` (the same for every lane of the wave) broadcast to all
 // lanes through a scalar register; __shfl would use an LDS permute.
 __device__ __forceinline__ int lane_bcast(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
+// Lane `dst` (the same for every lane) of a and b set to the wave-uniform
+// va and vb: one v_writelane_b32 each, which write that lane whatever EXEC
+// holds (instead of a compare and two selects). The lane select goes
+// through M0 (a VOP3 reads one SGPR on the constant bus), written by a SALU
+// move, which the lane selects may read after one wait state.
+__device__ __forceinline__ void lane_set2(int &a, int &b, int va, int vb, int dst)
+{
+    asm("s_mov_b32 m0, %4\n\ts_nop 0\n\tv_writelane_b32 %0, %2, m0\n\t"
+        "v_writelane_b32 %1, %3, m0"
+        : "+v"(a), "+v"(b)
+        : "s"(va), "s"(vb), "s"(dst)
+        : "m0");
+}
 __device__ __forceinline__ float lane_bcast(float v, int src)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
