@@ -21,7 +21,16 @@ struct AnalysisCounters {
     int nseg;                      // big-row segments
     int moved;                     // learned sizing: rows in a smaller table
     long long nnz_total;           // scan total
+    int n_long;                    // row statistics: rows left to the long-row pass
 };
+
+// One block's totals of the row-statistics passes (summed into
+// AnalysisCounters by a single block, instead of global atomics per block).
+struct StatPart {
+    int hist[16];
+    unsigned long long prod, cap;
+};
+constexpr int kStatsParts = 4096;  // partials a launch_row_stats call may write
 
 // Big-row state: bitmaps, chunk counts, windows, segments, cursors.
 struct BigArgs {
@@ -141,10 +150,13 @@ constexpr int64_t kFanMinProducts = int64_t(1) << 21;
 constexpr int64_t kFanMinFill = int64_t(1) << 24;
 
 // bst: k records of scratch, the B rows' (length, first column, last column)
-void launch_row_stats(hipStream_t st, int32_t m, const int32_t *arp, const int32_t *aci,
-                      int32_t k, const int32_t *brp, const int32_t *bci, int4 *bst, int64_t *ops,
-                      int32_t *lo, int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr,
-                      int hash_max_u);
+// long_rows: scratch of m ints (rows of more than 512 A entries, reduced by
+// a block each after the per-row pass); parts: kStatsParts partial totals
+void launch_row_stats(hipStream_t st, int32_t m, int32_t nnz_a, const int32_t *arp,
+                      const int32_t *aci, int32_t k, const int32_t *brp, const int32_t *bci,
+                      int4 *bst, int64_t *ops, int32_t *lo, int32_t *hi, uint8_t *sbin,
+                      AnalysisCounters *ctr, int hash_max_u, int32_t *long_rows,
+                      StatPart *parts);
 void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int32_t *off,
                         int32_t *cursor, int32_t *perm, int nb);
 // learned: hash bins use the overflow-checking count (a row whose count

@@ -491,6 +491,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     const size_t o_border = off; off = align256(off + kBigOrderWork * 4);
     const size_t o_err = off; off = align256(off + 4);
     const size_t o_bst = off; off = align256(off + (size_t)k * sizeof(int4));
+    const size_t o_parts = off; off = align256(off + (size_t)kStatsParts * sizeof(StatPart));
     rvsp_status_t s = palloc(p, &p->rowblock, off);
     if (s != RVSP_SUCCESS) return s;
     char *base = (char *)p->rowblock;
@@ -511,6 +512,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     int32_t *border = (int32_t *)(base + o_border);
     p->err = (int *)(base + o_err);
     int4 *bst = (int4 *)(base + o_bst);
+    StatPart *parts = (StatPart *)(base + o_parts);
 
     PhaseScope timer(st, PH_ANALYSE);
     HIP_TRY(hipMemsetAsync(p->ctr, 0, sizeof(AnalysisCounters), st));
@@ -547,8 +549,10 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->sized_rows = p->fallback_rows = 0;
 
     // ---- 1. per-row product counts, spans and symbolic bins
-    launch_row_stats(st, m, arp, aci, k, brp, bci, bst, p->ops, p->lo, p->hi, p->sbin, p->ctr,
-                     p->tp.hash_max_u);
+    // (the symbolic row order, filled later, lends its m ints to the list of
+    // long rows)
+    launch_row_stats(st, m, nnz_a, arp, aci, k, brp, bci, bst, p->ops, p->lo, p->hi, p->sbin,
+                     p->ctr, p->tp.hash_max_u, p->sperm, parts);
     HIP_TRY(hipGetLastError());
     if (sizer) {
         launch_sel_bins(st, m, sizer, margin, p->ops, arp, p->lo, p->hi, p->sbin, p->hbin, p->ctr);

@@ -72,7 +72,31 @@ __device__ __forceinline__ void row_stats_step(const int32_t *__restrict__ aci,
 // per row (sixteen rows per wave) take the first 32 entries of every row in
 // one step; the entries past that of the wave's longer rows are then taken
 // one row at a time by the whole wave, eight entries per lane per step.
+// Rows of more than kStatsLong entries are listed instead and reduced by a
+// block each in k_row_stats_long: a skewed A keeps its longest rows at low
+// indices (R-MAT 2^18: 15 744, then 6 600-entry rows), and one wave walking
+// several of them in turn was the whole kernel's time. The blocks stride
+// over groups of kStatsRowsPerBlock rows and leave their bin histogram,
+// product and bitmap-word totals in a StatPart of their own, which
+// k_stats_reduce sums: a global atomic per block and counter put every
+// block of a 2^20-row A on the same few L2 addresses (mixed 2^20 x 2^16:
+// 300 us for the pass).
+static_assert(SB_COUNT <= 16, "a StatPart holds 16 bins");
 constexpr int kStatsRowsPerBlock = 64;
+constexpr int kStatsBlocks = kStatsParts / 2;  // per pass: the two share the partials
+constexpr int kStatsLong = 512;
+constexpr int kStatsLongThreads = 512;
+
+// A row's product count and span, as both statistics kernels write them.
+__device__ __forceinline__ void row_stats_finish(int64_t row, int64_t ops, int32_t lo, int32_t hi,
+                                                 int64_t *__restrict__ ops_out,
+                                                 int32_t *__restrict__ lo_out,
+                                                 int32_t *__restrict__ hi_out)
+{
+    ops_out[row] = ops;
+    lo_out[row] = ops ? lo : 0;
+    hi_out[row] = ops ? hi : -1;
+}
 
 __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__restrict__ arp,
                                                    const int32_t *__restrict__ aci,
@@ -82,95 +106,219 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
                                                    int32_t *__restrict__ hi_out,
                                                    uint8_t *__restrict__ sbin_out,
                                                    AnalysisCounters *__restrict__ ctr,
-                                                   int hash_max_u)
+                                                   int hash_max_u,
+                                                   int32_t *__restrict__ long_rows,
+                                                   StatPart *__restrict__ parts)
 {
     constexpr int G = 4, U = 8;
     __shared__ int hist[SB_COUNT];
-    __shared__ unsigned long long bprod, bcap;  // block totals: one global atomic each
+    __shared__ unsigned long long bprod, bcap;
     if (threadIdx.x < SB_COUNT) hist[threadIdx.x] = 0;
     if (threadIdx.x == 0) bprod = bcap = 0ull;
     __syncthreads();
     const int lane = threadIdx.x & 63, gl = lane % G;
-    const int64_t row = (int64_t)blockIdx.x * kStatsRowsPerBlock + threadIdx.x / G;
-    int64_t ops = 0;
-    int32_t lo = INT32_MAX, hi = -1;
-    int as = 0, ae = 0;
-    if (row < m) {
-        as = arp[row];
-        ae = arp[row + 1];
-    }
-    const int e1 = min(ae, as + G * U);
-    row_stats_step<U>(aci, bst, as + gl, G, e1, ops, lo, hi);
-    uint64_t longs = __ballot(gl == 0 && ae > e1);
-    while (longs) {
-        const int src = __ffsll((unsigned long long)longs) - 1;
-        longs &= longs - 1;
-        const int ls = lane_bcast(e1, src), le = lane_bcast(ae, src);
-        int64_t o2 = 0;
-        int32_t lo2 = INT32_MAX, hi2 = -1;
-        for (int ap0 = ls + lane; ap0 < le; ap0 += U * 64)
-            row_stats_step<U>(aci, bst, ap0, 64, le, o2, lo2, hi2);
+    const int64_t ngroups = ((int64_t)m + kStatsRowsPerBlock - 1) / kStatsRowsPerBlock;
+    for (int64_t rg = blockIdx.x; rg < ngroups; rg += gridDim.x) {  // uniform over the block
+        const int64_t row = rg * kStatsRowsPerBlock + threadIdx.x / G;
+        int64_t ops = 0;
+        int32_t lo = INT32_MAX, hi = -1;
+        int as = 0, ae = 0;
+        if (row < m) {
+            as = arp[row];
+            ae = arp[row + 1];
+        }
+        const bool lng = ae - as > kStatsLong;  // the row's four lanes agree
+        const uint64_t lmask = __ballot(gl == 0 && lng);
+        if (lmask) {  // one list slot range per wave
+            int base = 0;
+            if (lane == __ffsll((unsigned long long)lmask) - 1)
+                base = atomicAdd(&ctr->n_long, __popcll(lmask));
+            base = __builtin_amdgcn_readlane(base, __ffsll((unsigned long long)lmask) - 1);
+            if (gl == 0 && lng)
+                long_rows[base + __popcll(lmask & ((1ull << lane) - 1ull))] = (int32_t)row;
+        }
+        if (lng) ae = as;
+        const int e1 = min(ae, as + G * U);
+        row_stats_step<U>(aci, bst, as + gl, G, e1, ops, lo, hi);
+        uint64_t longs = __ballot(gl == 0 && ae > e1);
+        while (longs) {
+            const int src = __ffsll((unsigned long long)longs) - 1;
+            longs &= longs - 1;
+            const int ls = lane_bcast(e1, src), le = lane_bcast(ae, src);
+            int64_t o2 = 0;
+            int32_t lo2 = INT32_MAX, hi2 = -1;
+            for (int ap0 = ls + lane; ap0 < le; ap0 += U * 64)
+                row_stats_step<U>(aci, bst, ap0, 64, le, o2, lo2, hi2);
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            o2 += __shfl_xor(o2, off, 64);
-            lo2 = min(lo2, __shfl_xor(lo2, off, 64));
-            hi2 = max(hi2, __shfl_xor(hi2, off, 64));
+            for (int off = 32; off > 0; off >>= 1) {
+                o2 += __shfl_xor(o2, off, 64);
+                lo2 = min(lo2, __shfl_xor(lo2, off, 64));
+                hi2 = max(hi2, __shfl_xor(hi2, off, 64));
+            }
+            if (lane == src) {
+                ops += o2;
+                lo = min(lo, lo2);
+                hi = max(hi, hi2);
+            }
         }
-        if (lane == src) {
-            ops += o2;
-            lo = min(lo, lo2);
-            hi = max(hi, hi2);
-        }
-    }
 #pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-        ops += __shfl_xor(ops, off, G);
-        lo = min(lo, __shfl_xor(lo, off, G));
-        hi = max(hi, __shfl_xor(hi, off, G));
-    }
-    if (row < m && gl == 0) {
-        const int b = sym_bin_of(ops, lo, hi, hash_max_u);
-        ops_out[row] = ops;
-        lo_out[row] = ops ? lo : 0;
-        hi_out[row] = ops ? hi : -1;
-        sbin_out[row] = (uint8_t)b;
-        atomicAdd(&hist[b], 1);
-        if (b == SB_BIG) {
-            atomicAdd(&bcap, (unsigned long long)big_nchunks(lo, hi) * kChunkWords);
+        for (int off = G / 2; off > 0; off >>= 1) {
+            ops += __shfl_xor(ops, off, G);
+            lo = min(lo, __shfl_xor(lo, off, G));
+            hi = max(hi, __shfl_xor(hi, off, G));
         }
-        if (ops) atomicAdd(&bprod, (unsigned long long)ops);
+        if (row < m && gl == 0 && !lng) {
+            const int b = sym_bin_of(ops, lo, hi, hash_max_u);
+            row_stats_finish(row, ops, lo, hi, ops_out, lo_out, hi_out);
+            sbin_out[row] = (uint8_t)b;
+            atomicAdd(&hist[b], 1);
+            if (b == SB_BIG) atomicAdd(&bcap, (unsigned long long)big_nchunks(lo, hi) * kChunkWords);
+            if (ops) atomicAdd(&bprod, (unsigned long long)ops);
+        }
     }
     __syncthreads();
-    if (threadIdx.x < SB_COUNT && hist[threadIdx.x])
-        atomicAdd(&ctr->sym_hist[threadIdx.x], hist[threadIdx.x]);
+    StatPart &pp = parts[blockIdx.x];
+    if (threadIdx.x < SB_COUNT) pp.hist[threadIdx.x] = hist[threadIdx.x];
     if (threadIdx.x == 0) {
-        if (bprod) atomicAdd(&ctr->products, bprod);
-        if (bcap) atomicAdd(&ctr->bits_cap, bcap);
+        pp.prod = bprod;
+        pp.cap = bcap;
+    }
+}
+
+// The listed long rows, a block of kStatsLongThreads each (the grid strides
+// over the list, whose length the per-row pass left in ctr->n_long); block
+// b's totals go to parts[b].
+__global__ __launch_bounds__(kStatsLongThreads) void k_row_stats_long(
+    const int32_t *__restrict__ arp, const int32_t *__restrict__ aci, const int4 *__restrict__ bst,
+    const int32_t *__restrict__ long_rows, int64_t *__restrict__ ops_out,
+    int32_t *__restrict__ lo_out, int32_t *__restrict__ hi_out, uint8_t *__restrict__ sbin_out,
+    const AnalysisCounters *__restrict__ ctr, int hash_max_u, StatPart *__restrict__ parts)
+{
+    constexpr int U = 8, NWV = kStatsLongThreads / 64;
+    __shared__ long long s_ops[NWV];
+    __shared__ int s_lo[NWV], s_hi[NWV];
+    __shared__ int hist[SB_COUNT];
+    if (threadIdx.x < SB_COUNT) hist[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned long long bprod = 0, bcap = 0;  // thread 0's
+    const int n = ctr->n_long;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {  // uniform over the block
+        const int row = long_rows[i];
+        const int as = arp[row], ae = arp[row + 1];
+        int64_t ops = 0;
+        int32_t lo = INT32_MAX, hi = -1;
+        for (int ap0 = as + (int)threadIdx.x; ap0 < ae; ap0 += U * kStatsLongThreads)
+            row_stats_step<U>(aci, bst, ap0, kStatsLongThreads, ae, ops, lo, hi);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            ops += __shfl_xor(ops, off, 64);
+            lo = min(lo, __shfl_xor(lo, off, 64));
+            hi = max(hi, __shfl_xor(hi, off, 64));
+        }
+        if (lane == 0) {
+            s_ops[wv] = ops;
+            s_lo[wv] = lo;
+            s_hi[wv] = hi;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < NWV; ++w) {
+                ops += s_ops[w];
+                lo = min(lo, s_lo[w]);
+                hi = max(hi, s_hi[w]);
+            }
+            const int b = sym_bin_of(ops, lo, hi, hash_max_u);
+            row_stats_finish(row, ops, lo, hi, ops_out, lo_out, hi_out);
+            sbin_out[row] = (uint8_t)b;
+            ++hist[b];
+            if (b == SB_BIG) bcap += (unsigned long long)big_nchunks(lo, hi) * kChunkWords;
+            bprod += (unsigned long long)ops;
+        }
+        __syncthreads();  // the block's partials are read before the next row's
+    }
+    StatPart &pp = parts[blockIdx.x];
+    if (threadIdx.x < SB_COUNT) pp.hist[threadIdx.x] = hist[threadIdx.x];
+    if (threadIdx.x == 0) {
+        pp.prod = bprod;
+        pp.cap = bcap;
+    }
+}
+
+// Sums the statistics passes' partials into the analysis counters (one
+// block; nothing else writes these counters before it).
+__global__ __launch_bounds__(256) void k_stats_reduce(const StatPart *__restrict__ parts, int n,
+                                                      AnalysisCounters *__restrict__ ctr)
+{
+    constexpr int NV = SB_COUNT + 2;
+    __shared__ unsigned long long s_v[4][NV];
+    unsigned long long v[NV];
+#pragma unroll
+    for (int b = 0; b < NV; ++b) v[b] = 0ull;
+    for (int i = threadIdx.x; i < n; i += 256) {
+#pragma unroll
+        for (int b = 0; b < SB_COUNT; ++b) v[b] += (unsigned long long)parts[i].hist[b];
+        v[SB_COUNT] += parts[i].prod;
+        v[SB_COUNT + 1] += parts[i].cap;
+    }
+#pragma unroll
+    for (int b = 0; b < NV; ++b)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[b] += __shfl_xor(v[b], off, 64);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int b = 0; b < NV; ++b) s_v[threadIdx.x >> 6][b] = v[b];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const unsigned long long t = s_v[0][threadIdx.x] + s_v[1][threadIdx.x] +
+                                     s_v[2][threadIdx.x] + s_v[3][threadIdx.x];
+        if (threadIdx.x < SB_COUNT)
+            ctr->sym_hist[threadIdx.x] = (int)t;
+        else if (threadIdx.x == SB_COUNT)
+            ctr->products = t;
+        else
+            ctr->bits_cap = t;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Stable-enough scatter of rows into per-bin lists (order inside a bin is
-// arbitrary; every row is processed independently, so results do not
-// depend on it).
+// Scatter of rows into per-bin lists (order inside a bin is arbitrary;
+// every row is processed independently, so results do not depend on it).
+// A block takes kScatterRows rows, thread t rows t, t + 256, ... (so each
+// atomic instruction places 64 consecutive rows and a bin's list stays
+// near row order, which neighbouring stencil rows' shared B rows in L2
+// reward), and counts them per bin in LDS before one global atomic per
+// bin: a block of 256 rows and an atomic per bin each put every block of a
+// 2^20-row matrix on the same few L2 addresses (50 us for the mixed
+// 2^20 x 2^16 product).
 // ---------------------------------------------------------------------------
+constexpr int kScatterRows = 4096;
+
 __global__ __launch_bounds__(256) void k_bin_scatter(int32_t m, const uint8_t *__restrict__ bin,
                                                      const int32_t *__restrict__ off,
                                                      int32_t *__restrict__ cursor,
                                                      int32_t *__restrict__ perm, int nb)
 {
+    constexpr int PER = kScatterRows / 256;
     __shared__ int lcount[64];
     __shared__ int lbase[64];
     if (threadIdx.x < 64) lcount[threadIdx.x] = 0;
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int b = i < m ? (int)bin[i] : -1;
-    const int lpos = b >= 0 ? atomicAdd(&lcount[b], 1) : 0;
+    const int64_t r0 = (int64_t)blockIdx.x * kScatterRows + threadIdx.x;
+    uint8_t b[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) b[u] = r0 + u * 256 < m ? bin[r0 + u * 256] : (uint8_t)0xFF;
+    int lpos[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) lpos[u] = b[u] != 0xFF ? atomicAdd(&lcount[b[u]], 1) : 0;
     __syncthreads();
     if (threadIdx.x < nb && lcount[threadIdx.x] > 0)
-        lbase[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], lcount[threadIdx.x]);
+        lbase[threadIdx.x] = off[threadIdx.x] + atomicAdd(&cursor[threadIdx.x], lcount[threadIdx.x]);
     __syncthreads();
-    if (b >= 0) perm[off[b] + lbase[b] + lpos] = (int32_t)i;
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+        if (b[u] != 0xFF) perm[lbase[b[u]] + lpos[u]] = (int32_t)(r0 + u * 256);
 }
 
 // ---------------------------------------------------------------------------
@@ -719,21 +867,32 @@ static void launch_sym_bins(hipStream_t main, const SymArgs &a, const int32_t *p
     }
 }
 
-void launch_row_stats(hipStream_t st, int32_t m, const int32_t *arp, const int32_t *aci,
-                      int32_t k, const int32_t *brp, const int32_t *bci, int4 *bst, int64_t *ops,
-                      int32_t *lo, int32_t *hi, uint8_t *sbin, AnalysisCounters *ctr,
-                      int hash_max_u)
+void launch_row_stats(hipStream_t st, int32_t m, int32_t nnz_a, const int32_t *arp,
+                      const int32_t *aci, int32_t k, const int32_t *brp, const int32_t *bci,
+                      int4 *bst, int64_t *ops, int32_t *lo, int32_t *hi, uint8_t *sbin,
+                      AnalysisCounters *ctr, int hash_max_u, int32_t *long_rows,
+                      StatPart *parts)
 {
     k_brow_stats<<<(int)(((int64_t)k + 255) / 256), 256, 0, st>>>(k, brp, bci, bst);
-    const int grid = (int)(((int64_t)m + kStatsRowsPerBlock - 1) / kStatsRowsPerBlock);
+    const int64_t ngroups = ((int64_t)m + kStatsRowsPerBlock - 1) / kStatsRowsPerBlock;
+    const int g1 = (int)std::max<int64_t>(1, std::min<int64_t>(ngroups, kStatsBlocks));
     const int hmax = hash_max_u < 64 ? 64 : hash_max_u > kSymHashMaxU ? kSymHashMaxU : hash_max_u;
-    k_row_stats<<<grid, 256, 0, st>>>(m, arp, aci, bst, ops, lo, hi, sbin, ctr, hmax);
+    k_row_stats<<<g1, 256, 0, st>>>(m, arp, aci, bst, ops, lo, hi, sbin, ctr, hmax, long_rows,
+                                    parts);
+    int g2 = 0;
+    if (nnz_a > kStatsLong) {  // else no row can be long
+        g2 = std::min(nnz_a / (kStatsLong + 1), kStatsBlocks);  // listed rows at most
+        k_row_stats_long<<<g2, kStatsLongThreads, 0, st>>>(arp, aci, bst, long_rows, ops, lo, hi,
+                                                           sbin, ctr, hmax, parts + g1);
+    }
+    k_stats_reduce<<<1, 256, 0, st>>>(parts, g1 + g2, ctr);
 }
 
 void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int32_t *off,
                         int32_t *cursor, int32_t *perm, int nb)
 {
-    const int grid = (int)(((int64_t)m + 255) / 256);
+    if (m <= 0) return;
+    const int grid = (int)(((int64_t)m + kScatterRows - 1) / kScatterRows);
     k_bin_scatter<<<grid, 256, 0, st>>>(m, bin, off, cursor, perm, nb);
 }
 
