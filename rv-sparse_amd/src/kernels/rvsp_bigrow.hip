@@ -54,16 +54,20 @@ static_assert(kRegGroups >= 1 && kRegGroups <= 4, "register cursors: one round o
 constexpr int kSymLong = 48;    // B rows of more pairs than this are shared by a wave
 constexpr int kSymPiece = 512;  // pairs per wave piece (8 per lane)
 
-// LDS slot of bitmap word w: its low six bits XORed with a multiplicative
-// hash of the rest. Skewed column distributions (R-MAT's bits are each 0
-// with probability a + c) crowd the words that share their low bits, and
-// so one LDS bank; the hash spreads them. A bijection on every 64-word
-// block, so the bitmap needs its word count rounded up to 64.
+// LDS slot of bitmap word w: its low six bits XORed with two shifted fields
+// of the rest. Skewed column distributions (R-MAT's bits are each 0 with
+// probability a + c) crowd the words that share their low bits, and so one
+// LDS bank; the fields spread them about as well as a multiplicative hash
+// (64-word batches of R-MAT 2^18's B rows: 4.44 lanes on the fullest of 32
+// banks against 4.35, and 6.07 unhashed) with two bit-field extracts and
+// two XORs instead of a quarter-rate 32-bit multiply. A bijection on every
+// 64-word block, so the bitmap needs its word count rounded up to 64.
 static_assert(kBigPassWords % 64 == 0, "a full bitmap pass is whole 64-word blocks");
 
 __device__ __forceinline__ int lds_word(int w)
 {
-    return w ^ (int)(((uint32_t)(w >> 6) * 2654435761u) >> 26);
+    const uint32_t u = (uint32_t)w;
+    return w ^ (int)(__builtin_amdgcn_ubfe(u, 6u, 6u) ^ __builtin_amdgcn_ubfe(u, 11u, 6u));
 }
 
 template <int NT>
@@ -136,14 +140,18 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
                     else
                         hi = mid - 1;
                 }
-                const int bs = s_bs[lo] + (pc - s_pfx[lo]) * kSymPiece;
-                const int be = min(bs + kSymPiece, s_be[lo]);
+                // wave-uniform (pc is): held in scalar registers
+                const int bs =
+                    __builtin_amdgcn_readfirstlane(s_bs[lo] + (pc - s_pfx[lo]) * kSymPiece);
+                const int be = __builtin_amdgcn_readfirstlane(min(bs + kSymPiece, s_be[lo]));
+                // wave-uniform base and 32-bit lane offsets: the loads take
+                // the scalar-base address form
+                const int2 *__restrict__ at = g.bwm + bs;
+                const int left = be - bs;
                 int2 c[kSymPiece / 64];
 #pragma unroll
-                for (int u = 0; u < kSymPiece / 64; ++u) {
-                    const int bp = bs + u * 64 + lane;
-                    c[u] = bp < be ? g.bwm[bp] : make_int2(-1, 0);
-                }
+                for (int u = 0; u < kSymPiece / 64; ++u)
+                    c[u] = u * 64 + lane < left ? at[u * 64 + lane] : make_int2(-1, 0);
 #pragma unroll
                 for (int u = 0; u < kSymPiece / 64; ++u) {
                     const uint32_t rel = (uint32_t)(c[u].x - wbase);
