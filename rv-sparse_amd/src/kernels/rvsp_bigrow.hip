@@ -381,11 +381,28 @@ __global__ __launch_bounds__(NWV * 64) void k_big_fill(int32_t nsegs, BigArgs g,
 // wave, so the widest rows spread over many waves; their cursor blocks
 // (alen entries each) are carved from a running counter.
 // ---------------------------------------------------------------------------
+// Four ints read as one 16-byte access at 4-byte alignment.
+struct __attribute__((aligned(4))) I4 {
+    int x, y, z, w;
+};
+
+// The segment list holds the rows' segments one row after another in the
+// rows' list order, largest first, so the numeric pass starts the longest
+// rows first. (Reserving each block's range with an atomic left the order
+// to the blocks' finishing times: R-MAT 2^18's numeric pass took 15.8 to
+// 16.8 ms depending on it, 18.7 ms with the smallest rows first, 15.7 ms
+// largest first; generated 32768^2 6.0 to 7.5 ms, 5.45 ms largest first.)
+//
+// EMIT false: windows, then each row's segment count (segcnt[j]); EMIT
+// true: each row's segments at its place in the list, from the scanned
+// counts (segoff, nrows + 1 entries).
+template <bool EMIT>
 __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ rows, int32_t nrows,
                                                   BigArgs g, const int32_t *__restrict__ crp,
-                                                  AnalysisCounters *__restrict__ ctr)
+                                                  AnalysisCounters *__restrict__ ctr,
+                                                  int32_t *__restrict__ segcnt,
+                                                  const int32_t *__restrict__ segoff)
 {
-    __shared__ int s_seg;            // block's first segment index
     __shared__ long long s_cur;      // block's first cursor entry
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool valid = j < nrows;    // every thread joins the block scans below
@@ -397,19 +414,35 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
     // products per output of this row: segments aim at ~g.seg_products each
     const double ratio = nz > 0 ? (double)g.ops[row] / (double)nz : 1.0;
     int nwin = 0;
-    if (valid) {
+    if (EMIT) {
+        if (valid) nwin = g.big_nwin[row];
+    } else if (valid) {
         int out = 0, acc = 0, start = 0;
-        // pass 1: windows (chunk counts read eight per round trip)
-        for (int c0 = 0; c0 < nchunks; c0 += 8) {
-            int cc8[8];
+        // pass 1: windows (chunk counts read 16 bytes at a time, eight per
+        // round trip: a thread walks its own row, so each load is a line of
+        // its own; the target allows 16-byte loads at 4-byte alignment, and
+        // a load that starts inside the row's counts ends at most 12 bytes
+        // past them, inside the big-row allocation, where the window arrays
+        // follow the counts and the segments follow the window offsets;
+        // entries past the row are not used)
+        for (int c0 = 0; c0 < nchunks; c0 += 32) {
+            int cc[32];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) cc8[u] = c0 + u < nchunks ? g.gchunk[c32 + c0 + u] : 0;
+            for (int q = 0; q < 8; ++q) {
+                const I4 v = c0 + 4 * q < nchunks
+                                 ? *reinterpret_cast<const I4 *>(g.gchunk + c32 + c0 + 4 * q)
+                                 : I4{0, 0, 0, 0};
+                cc[4 * q] = v.x;
+                cc[4 * q + 1] = v.y;
+                cc[4 * q + 2] = v.z;
+                cc[4 * q + 3] = v.w;
+            }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 32; ++u) {
                 const int c = c0 + u;
                 if (c >= nchunks) break;
-                const int cc = cc8[u];
-                if (c > start && (acc + cc > g.win_outs || c - start >= g.win_chunks)) {
+                const int cnt_c = cc[u];
+                if (c > start && (acc + cnt_c > g.win_outs || c - start >= g.win_chunks)) {
                     g.win_chunk0[c32 + nwin] = start;
                     g.win_out[c32 + nwin] = out;
                     ++nwin;
@@ -417,7 +450,7 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
                     start = c;
                     acc = 0;
                 }
-                acc += cc;
+                acc += cnt_c;
             }
         }
         g.win_chunk0[c32 + nwin] = start;
@@ -425,43 +458,51 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
         ++nwin;
         g.big_nwin[row] = nwin;
     }
-    // pass 2: segments, counted, then ranges reserved for the whole block
-    // with one atomic each, then emitted
-    int nseg = 0;
-    for (int pass = 0; pass < 2; ++pass) {
+    // pass 2: segments, counted (EMIT false), or emitted at the row's place
+    // in the list with cursor ranges reserved for the whole block by one
+    // atomic (EMIT true)
+    {
         int sbase = 0;
         long long cbase = 0;
-        if (pass == 1) {
-            int64_t tseg, tcur;
-            const int64_t xseg = block_excl_scan(nseg, &tseg);
+        if (EMIT) {
+            const int nseg = valid ? segoff[j + 1] - segoff[j] : 0;
+            int64_t tcur;
             // rows short enough for register cursors reserve none
             const int64_t need = alen > kRegGroups * 64 ? (int64_t)alen : 0;
-            const int64_t xcur = block_excl_scan(need * nseg * (valid ? 1 : 0), &tcur);
-            if (threadIdx.x == 0) {
-                s_seg = tseg ? atomicAdd(&ctr->nseg, (int)tseg) : 0;
+            const int64_t xcur = block_excl_scan(need * nseg, &tcur);
+            if (threadIdx.x == 0)
                 s_cur = tcur ? (long long)atomicAdd(&ctr->cur_total, (unsigned long long)tcur) : 0;
-            }
             __syncthreads();
-            sbase = s_seg + (int)xseg;
+            if (valid) sbase = segoff[j];
             cbase = s_cur + xcur;
         }
         int s = 0, w0 = 0;
         double est = 0.0;
-        // window output offsets read eight per round trip (plus the next one)
-        for (int wb = 0; wb < nwin; wb += 8) {
-            int o8[9];
+        // window output offsets, 16 bytes a load, 32 per round trip (the
+        // next one, the first of the following batch, read alone)
+        for (int wb = 0; wb < nwin; wb += 32) {
+            int ow[33];
 #pragma unroll
-            for (int u = 0; u < 9; ++u) {
-                const int w = wb + u;
-                o8[u] = w < nwin ? g.win_out[c32 + w] : nz;
+            for (int q = 0; q < 8; ++q) {
+                const I4 v = wb + 4 * q < nwin
+                                 ? *reinterpret_cast<const I4 *>(g.win_out + c32 + wb + 4 * q)
+                                 : I4{nz, nz, nz, nz};
+                ow[4 * q] = v.x;
+                ow[4 * q + 1] = v.y;
+                ow[4 * q + 2] = v.z;
+                ow[4 * q + 3] = v.w;
             }
+            ow[32] = wb + 32 < nwin ? g.win_out[c32 + wb + 32] : nz;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 33; ++u)
+                if (wb + u >= nwin) ow[u] = nz;
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
                 const int w = wb + u;
                 if (w >= nwin) break;
-                est += (o8[u + 1] - o8[u]) * ratio;
+                est += (ow[u + 1] - ow[u]) * ratio;
                 if (est >= (double)g.seg_products || w == nwin - 1) {
-                    if (pass == 1) {
+                    if (EMIT) {
                         g.segs[sbase + s] = make_int4(row, w0, w + 1, 0);
                         g.seg_cur[sbase + s] = cbase + (long long)s * (alen > kRegGroups * 64 ? alen : 0);
                     }
@@ -471,7 +512,7 @@ __global__ __launch_bounds__(256) void k_big_plan(const int32_t *__restrict__ ro
                 }
             }
         }
-        nseg = s;
+        if (!EMIT && valid) segcnt[j] = s;
     }
 }
 
@@ -971,9 +1012,14 @@ void launch_big_fill(hipStream_t st, const BigArgs &g, const int32_t *crp, int32
 }
 
 void launch_big_plan(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,
-                     const int32_t *crp, AnalysisCounters *ctr)
+                     const int32_t *crp, AnalysisCounters *ctr, int32_t *segcnt, int32_t *segoff,
+                     int64_t *tsum)
 {
-    if (nrows > 0) k_big_plan<<<(nrows + 255) / 256, 256, 0, st>>>(rows, nrows, g, crp, ctr);
+    if (nrows <= 0) return;
+    const int grid = (nrows + 255) / 256;
+    k_big_plan<false><<<grid, 256, 0, st>>>(rows, nrows, g, crp, ctr, segcnt, segoff);
+    launch_scan(st, nrows, segcnt, tsum, (int64_t *)&ctr->seg_total, segoff);
+    k_big_plan<true><<<grid, 256, 0, st>>>(rows, nrows, g, crp, ctr, segcnt, segoff);
 }
 
 template <int PW, int MAXCH, class Args>

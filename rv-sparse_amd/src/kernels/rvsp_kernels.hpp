@@ -18,10 +18,10 @@ struct AnalysisCounters {
     unsigned long long bits_cap;   // sum over big rows of bitmap words
     unsigned long long bits_fill;  // running allocation of bitmap words
     unsigned long long cur_total;  // cursors over all big-row segments
-    int nseg;                      // big-row segments
     int moved;                     // learned sizing: rows in a smaller table
     long long nnz_total;           // scan total
     int n_long;                    // row statistics: rows left to the long-row pass
+    long long seg_total;           // big-row segments (scan of the per-row counts)
 };
 
 // One block's totals of the row-statistics passes (summed into
@@ -197,8 +197,10 @@ void launch_big_words(hipStream_t st, int32_t k, int32_t nnz, const int32_t *brp
                       int64_t *total, int2 *bwm, int32_t *bwp);
 // Big-row structure fill over every planned segment (g.nsegs).
 void launch_big_fill(hipStream_t st, const BigArgs &g, const int32_t *crp, int32_t *cci);
+// segcnt: nrows ints; segoff: nrows + 1 ints; tsum: scan tiles of nrows
 void launch_big_plan(hipStream_t st, const int32_t *rows, int nrows, const BigArgs &g,
-                     const int32_t *crp, AnalysisCounters *ctr);
+                     const int32_t *crp, AnalysisCounters *ctr, int32_t *segcnt, int32_t *segoff,
+                     int64_t *tsum);
 void launch_num_big(hipStream_t st, int nsegs, const NumArgs &a, const BigArgs &g);
 void launch_num_big(hipStream_t st, int nsegs, const NumArgsD &a, const BigArgs &g);
 void launch_num_big(hipStream_t st, int nsegs, const NumArgsI &a, const BigArgs &g);

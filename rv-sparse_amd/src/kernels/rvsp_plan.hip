@@ -492,6 +492,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     const size_t o_err = off; off = align256(off + 4);
     const size_t o_bst = off; off = align256(off + (size_t)k * sizeof(int4));
     const size_t o_parts = off; off = align256(off + (size_t)kStatsParts * sizeof(StatPart));
+    const size_t o_segoff = off; off = align256(off + (M + 1) * 4);
     rvsp_status_t s = palloc(p, &p->rowblock, off);
     if (s != RVSP_SUCCESS) return s;
     char *base = (char *)p->rowblock;
@@ -513,6 +514,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     p->err = (int *)(base + o_err);
     int4 *bst = (int4 *)(base + o_bst);
     StatPart *parts = (StatPart *)(base + o_parts);
+    int32_t *segoff = (int32_t *)(base + o_segoff);
 
     PhaseScope timer(st, PH_ANALYSE);
     HIP_TRY(hipMemsetAsync(p->ctr, 0, sizeof(AnalysisCounters), st));
@@ -720,8 +722,10 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     HIP_TRY(hipGetLastError());
     // ---- 4. big-row windows and segments, numeric bins
     if (p->sym_count[SB_BIG] > 0) {
+        // (nperm, free until the numeric bins are scattered, holds the
+        // per-row segment counts; the row-pointer scan's tiles are done)
         launch_big_plan(st, p->sperm + p->sym_off[SB_BIG], p->sym_count[SB_BIG], p->big, p->crp,
-                        p->ctr);
+                        p->ctr, p->nperm, segoff, p->tsum);
         HIP_TRY(hipGetLastError());
     }
     launch_num_bins(st, m, p->crp, p->ops, arp, p->sbin, p->lo, p->hi, chooser,
@@ -730,7 +734,7 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     HIP_TRY(hipMemcpyAsync(&p->hctr, p->ctr, sizeof(AnalysisCounters), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     p->nnzc = p->hctr.nnz_total;
-    p->nseg = p->hctr.nseg;
+    p->nseg = (int)p->hctr.seg_total;
     p->big.nsegs = p->nseg;
     if (p->nseg > 0) {
         s = palloc(p, (void **)&p->cursors, (size_t)p->hctr.cur_total * sizeof(int4));
