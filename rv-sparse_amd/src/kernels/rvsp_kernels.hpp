@@ -152,10 +152,12 @@ constexpr int64_t kFanMinFill = int64_t(1) << 24;
 // bst: k records of scratch, the B rows' (length, first column, last column)
 // long_rows: scratch of m ints (rows of more than 512 A entries, reduced by
 // a block each after the per-row pass); parts: kStatsParts partial totals
+// Every index the statistics read through is clamped, so they may run on
+// device operands whose check the host has not read yet.
 void launch_row_stats(hipStream_t st, int32_t m, int32_t nnz_a, const int32_t *arp,
-                      const int32_t *aci, int32_t k, const int32_t *brp, const int32_t *bci,
-                      int4 *bst, int64_t *ops, int32_t *lo, int32_t *hi, uint8_t *sbin,
-                      AnalysisCounters *ctr, int hash_max_u, int32_t *long_rows,
+                      const int32_t *aci, int32_t k, int32_t nnz_b, const int32_t *brp,
+                      const int32_t *bci, int4 *bst, int64_t *ops, int32_t *lo, int32_t *hi,
+                      uint8_t *sbin, AnalysisCounters *ctr, int hash_max_u, int32_t *long_rows,
                       StatPart *parts);
 void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int32_t *off,
                         int32_t *cursor, int32_t *perm, int nb);

@@ -522,7 +522,10 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     HIP_TRY(hipMemsetAsync(p->err, 0, 4, st));
 
     // ---- 0. operands the host has not checked: canonical CSR, on the
-    // device, before any kernel trusts their indices
+    // device, before any kernel trusts their indices. The host reads the
+    // verdict with the row statistics' counters (one synchronisation less):
+    // the statistics clamp every index they read through, and nothing else
+    // runs before the verdict is read.
     const bool checked = p->operands_checked;
     p->operands_checked = false;
     if (!checked) {
@@ -534,12 +537,8 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
         hipError_t e = launch_csr_validate(st, m, k, nnz_a, arp, aci, (uint32_t *)starts, p->err);
         if (e == hipSuccess && !same)
             e = launch_csr_validate(st, k, n, nnz_b, brp, bci, (uint32_t *)starts, p->err);
-        int herr = 0;
-        if (e == hipSuccess) e = hipMemcpyAsync(&herr, p->err, 4, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        pfree(p, starts);
+        pfree(p, starts);  // stream-ordered: reused only by later work on st
         HIP_TRY(e);
-        if (herr) return RVSP_ERROR_INVALID_CSR;
     }
 
     // learned selectors (null when none is loaded: heuristic rules)
@@ -553,15 +552,18 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     // ---- 1. per-row product counts, spans and symbolic bins
     // (the symbolic row order, filled later, lends its m ints to the list of
     // long rows)
-    launch_row_stats(st, m, nnz_a, arp, aci, k, brp, bci, bst, p->ops, p->lo, p->hi, p->sbin,
-                     p->ctr, p->tp.hash_max_u, p->sperm, parts);
+    launch_row_stats(st, m, nnz_a, arp, aci, k, nnz_b, brp, bci, bst, p->ops, p->lo, p->hi,
+                     p->sbin, p->ctr, p->tp.hash_max_u, p->sperm, parts);
     HIP_TRY(hipGetLastError());
     if (sizer) {
         launch_sel_bins(st, m, sizer, margin, p->ops, arp, p->lo, p->hi, p->sbin, p->hbin, p->ctr);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipMemcpyAsync(&p->hctr, p->ctr, sizeof(AnalysisCounters), hipMemcpyDeviceToHost, st));
+    int herr = 0;
+    if (!checked) HIP_TRY(hipMemcpyAsync(&herr, p->err, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (herr) return RVSP_ERROR_INVALID_CSR;
     p->products = (int64_t)p->hctr.products;
 
     int32_t *hoff = p->h_off[0];
@@ -732,7 +734,11 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
                     cmargin > 1.0f ? log2f(cmargin) : 0.0f, p->products, p->nbin, p->ctr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&p->hctr, p->ctr, sizeof(AnalysisCounters), hipMemcpyDeviceToHost, st));
+    // the counts' error word (a full hash table) comes back in the same
+    // synchronisation; no later analysis kernel raises one
+    HIP_TRY(hipMemcpyAsync(&herr, p->err, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (herr) return RVSP_ERROR_INVALID_CSR;
     p->nnzc = p->hctr.nnz_total;
     p->nseg = (int)p->hctr.seg_total;
     p->big.nsegs = p->nseg;
@@ -758,10 +764,6 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     HIP_TRY(hipMemsetAsync(p->binoff + 64, 0, 64 * 4, st));
     launch_bin_scatter(st, m, p->nbin, p->binoff, p->binoff + 64, p->nperm, NB_COUNT);
     HIP_TRY(hipGetLastError());
-    int herr = 0;
-    HIP_TRY(hipMemcpyAsync(&herr, p->err, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (herr) return RVSP_ERROR_INVALID_CSR;
     if (c_nnz) *c_nnz = p->nnzc;
     if (products) *products = p->products;
     if (p->nnzc > INT32_MAX) return RVSP_ERROR_ALLOCATION_FAILED;

@@ -32,34 +32,47 @@ __device__ __forceinline__ int sym_bin_of(int64_t ops, int32_t lo, int32_t hi, i
     return SB_BIG;
 }
 
+// Row r's entry range with every bound clamped into [0, nnz]: the row
+// statistics run before the host has read the device operands' check, so
+// their reads stay in bounds on operands that fail it (the host then
+// discards the statistics and reports the operands invalid).
+__device__ __forceinline__ int2 clamped_row(const int32_t *__restrict__ rp, int64_t r, int32_t nnz)
+{
+    const int s = min(max(rp[r], 0), nnz);
+    return make_int2(s, min(max(rp[r + 1], s), nnz));
+}
+
 // Per B row: its length, first column and last column (INT32_MAX and -1
 // when empty), so a row walk reads one 16-byte record per A entry instead
 // of the row bounds and then both end columns.
-__global__ __launch_bounds__(256) void k_brow_stats(int32_t k, const int32_t *__restrict__ brp,
+__global__ __launch_bounds__(256) void k_brow_stats(int32_t k, int32_t nnz_b,
+                                                    const int32_t *__restrict__ brp,
                                                     const int32_t *__restrict__ bci,
                                                     int4 *__restrict__ bst)
 {
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (r >= k) return;
-    const int bs = brp[r], be = brp[r + 1];
-    bst[r] = make_int4(be - bs, be > bs ? bci[bs] : INT32_MAX, be > bs ? bci[be - 1] : -1, 0);
+    const int2 b = clamped_row(brp, r, nnz_b);
+    bst[r] = make_int4(b.y - b.x, b.y > b.x ? bci[b.x] : INT32_MAX, b.y > b.x ? bci[b.y - 1] : -1, 0);
 }
 
 // Sums |B_k|, and takes the smallest first and largest last column of the
 // B_k, over the A entries ap0 + t * step (t < U, ap0 + t * step < end): the
 // two dependent loads (A column, B row record) are each issued for all U
 // entries before any is used.
+// (A columns outside [0, kb) are skipped: see clamped_row.)
 template <int U>
 __device__ __forceinline__ void row_stats_step(const int32_t *__restrict__ aci,
-                                               const int4 *__restrict__ bst, int ap0, int step,
-                                               int end, int64_t &ops, int32_t &lo, int32_t &hi)
+                                               const int4 *__restrict__ bst, int32_t kb, int ap0,
+                                               int step, int end, int64_t &ops, int32_t &lo,
+                                               int32_t &hi)
 {
     int k[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) k[u] = ap0 + u * step < end ? aci[ap0 + u * step] : -1;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        if (k[u] >= 0) {
+        if ((uint32_t)k[u] < (uint32_t)kb) {
             const int4 b = bst[k[u]];
             ops += b.x;
             lo = min(lo, b.y);
@@ -98,7 +111,8 @@ __device__ __forceinline__ void row_stats_finish(int64_t row, int64_t ops, int32
     hi_out[row] = ops ? hi : -1;
 }
 
-__global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__restrict__ arp,
+__global__ __launch_bounds__(256) void k_row_stats(int32_t m, int32_t nnz_a, int32_t kb,
+                                                   const int32_t *__restrict__ arp,
                                                    const int32_t *__restrict__ aci,
                                                    const int4 *__restrict__ bst,
                                                    int64_t *__restrict__ ops_out,
@@ -124,8 +138,9 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
         int32_t lo = INT32_MAX, hi = -1;
         int as = 0, ae = 0;
         if (row < m) {
-            as = arp[row];
-            ae = arp[row + 1];
+            const int2 r = clamped_row(arp, row, nnz_a);
+            as = r.x;
+            ae = r.y;
         }
         const bool lng = ae - as > kStatsLong;  // the row's four lanes agree
         const uint64_t lmask = __ballot(gl == 0 && lng);
@@ -139,7 +154,7 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
         }
         if (lng) ae = as;
         const int e1 = min(ae, as + G * U);
-        row_stats_step<U>(aci, bst, as + gl, G, e1, ops, lo, hi);
+        row_stats_step<U>(aci, bst, kb, as + gl, G, e1, ops, lo, hi);
         uint64_t longs = __ballot(gl == 0 && ae > e1);
         while (longs) {
             const int src = __ffsll((unsigned long long)longs) - 1;
@@ -148,7 +163,7 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
             int64_t o2 = 0;
             int32_t lo2 = INT32_MAX, hi2 = -1;
             for (int ap0 = ls + lane; ap0 < le; ap0 += U * 64)
-                row_stats_step<U>(aci, bst, ap0, 64, le, o2, lo2, hi2);
+                row_stats_step<U>(aci, bst, kb, ap0, 64, le, o2, lo2, hi2);
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
                 o2 += __shfl_xor(o2, off, 64);
@@ -189,7 +204,7 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, const int32_t *__r
 // over the list, whose length the per-row pass left in ctr->n_long); block
 // b's totals go to parts[b].
 __global__ __launch_bounds__(kStatsLongThreads) void k_row_stats_long(
-    const int32_t *__restrict__ arp, const int32_t *__restrict__ aci, const int4 *__restrict__ bst,
+    int32_t nnz_a, int32_t kb, const int32_t *__restrict__ arp, const int32_t *__restrict__ aci, const int4 *__restrict__ bst,
     const int32_t *__restrict__ long_rows, int64_t *__restrict__ ops_out,
     int32_t *__restrict__ lo_out, int32_t *__restrict__ hi_out, uint8_t *__restrict__ sbin_out,
     const AnalysisCounters *__restrict__ ctr, int hash_max_u, StatPart *__restrict__ parts)
@@ -205,11 +220,11 @@ __global__ __launch_bounds__(kStatsLongThreads) void k_row_stats_long(
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {  // uniform over the block
         const int row = long_rows[i];
-        const int as = arp[row], ae = arp[row + 1];
+        const int2 r = clamped_row(arp, row, nnz_a);
         int64_t ops = 0;
         int32_t lo = INT32_MAX, hi = -1;
-        for (int ap0 = as + (int)threadIdx.x; ap0 < ae; ap0 += U * kStatsLongThreads)
-            row_stats_step<U>(aci, bst, ap0, kStatsLongThreads, ae, ops, lo, hi);
+        for (int ap0 = r.x + (int)threadIdx.x; ap0 < r.y; ap0 += U * kStatsLongThreads)
+            row_stats_step<U>(aci, bst, kb, ap0, kStatsLongThreads, r.y, ops, lo, hi);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             ops += __shfl_xor(ops, off, 64);
@@ -868,22 +883,22 @@ static void launch_sym_bins(hipStream_t main, const SymArgs &a, const int32_t *p
 }
 
 void launch_row_stats(hipStream_t st, int32_t m, int32_t nnz_a, const int32_t *arp,
-                      const int32_t *aci, int32_t k, const int32_t *brp, const int32_t *bci,
-                      int4 *bst, int64_t *ops, int32_t *lo, int32_t *hi, uint8_t *sbin,
-                      AnalysisCounters *ctr, int hash_max_u, int32_t *long_rows,
+                      const int32_t *aci, int32_t k, int32_t nnz_b, const int32_t *brp,
+                      const int32_t *bci, int4 *bst, int64_t *ops, int32_t *lo, int32_t *hi,
+                      uint8_t *sbin, AnalysisCounters *ctr, int hash_max_u, int32_t *long_rows,
                       StatPart *parts)
 {
-    k_brow_stats<<<(int)(((int64_t)k + 255) / 256), 256, 0, st>>>(k, brp, bci, bst);
+    if (k > 0) k_brow_stats<<<(int)(((int64_t)k + 255) / 256), 256, 0, st>>>(k, nnz_b, brp, bci, bst);
     const int64_t ngroups = ((int64_t)m + kStatsRowsPerBlock - 1) / kStatsRowsPerBlock;
     const int g1 = (int)std::max<int64_t>(1, std::min<int64_t>(ngroups, kStatsBlocks));
     const int hmax = hash_max_u < 64 ? 64 : hash_max_u > kSymHashMaxU ? kSymHashMaxU : hash_max_u;
-    k_row_stats<<<g1, 256, 0, st>>>(m, arp, aci, bst, ops, lo, hi, sbin, ctr, hmax, long_rows,
-                                    parts);
+    k_row_stats<<<g1, 256, 0, st>>>(m, nnz_a, k, arp, aci, bst, ops, lo, hi, sbin, ctr, hmax,
+                                    long_rows, parts);
     int g2 = 0;
     if (nnz_a > kStatsLong) {  // else no row can be long
         g2 = std::min(nnz_a / (kStatsLong + 1), kStatsBlocks);  // listed rows at most
-        k_row_stats_long<<<g2, kStatsLongThreads, 0, st>>>(arp, aci, bst, long_rows, ops, lo, hi,
-                                                           sbin, ctr, hmax, parts + g1);
+        k_row_stats_long<<<g2, kStatsLongThreads, 0, st>>>(nnz_a, k, arp, aci, bst, long_rows, ops,
+                                                           lo, hi, sbin, ctr, hmax, parts + g1);
     }
     k_stats_reduce<<<1, 256, 0, st>>>(parts, g1 + g2, ctr);
 }
