@@ -132,14 +132,15 @@ __global__ __launch_bounds__(NT) void k_sym_big(const int32_t *__restrict__ rows
             }
             for (int pc = wv; pc < (int)npieces; pc += NWV) {
                 // entry e holding piece pc: the last e with s_pfx[e] <= pc
-                int lo = 0, hi = NT - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_pfx[mid] <= pc)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
-                }
+                // (s_pfx is non-decreasing). pc is wave-uniform, so the
+                // lanes search together: every S-th prefix, then the S
+                // prefixes of the block found, each a ballot whose set
+                // lanes are a prefix of the wave (one LDS read a level
+                // instead of a binary search's ten dependent rounds)
+                constexpr int S = NT / 64;
+                static_assert(NT % 64 == 0 && S <= 64, "two ballot levels cover the entries");
+                const int b0 = (__popcll(__ballot(s_pfx[lane * S] <= pc)) - 1) * S;
+                const int lo = b0 - 1 + __popcll(__ballot(lane < S && s_pfx[b0 + min(lane, S - 1)] <= pc));
                 // wave-uniform (pc is): held in scalar registers
                 const int bs =
                     __builtin_amdgcn_readfirstlane(s_bs[lo] + (pc - s_pfx[lo]) * kSymPiece);
