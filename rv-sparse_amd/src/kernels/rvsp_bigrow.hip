@@ -54,20 +54,19 @@ static_assert(kRegGroups >= 1 && kRegGroups <= 4, "register cursors: one round o
 constexpr int kSymLong = 48;    // B rows of more pairs than this are shared by a wave
 constexpr int kSymPiece = 512;  // pairs per wave piece (8 per lane)
 
-// LDS slot of bitmap word w: its low six bits XORed with two shifted fields
-// of the rest. Skewed column distributions (R-MAT's bits are each 0 with
-// probability a + c) crowd the words that share their low bits, and so one
-// LDS bank; the fields spread them about as well as a multiplicative hash
-// (64-word batches of R-MAT 2^18's B rows: 4.44 lanes on the fullest of 32
-// banks against 4.35, and 6.07 unhashed) with two bit-field extracts and
-// two XORs instead of a quarter-rate 32-bit multiply. A bijection on every
-// 64-word block, so the bitmap needs its word count rounded up to 64.
+// LDS slot of bitmap word w: its low six bits XORed with the next six.
+// Skewed column distributions (R-MAT's bits are each 0 with probability
+// a + c) crowd the words that share their low bits, and so one LDS bank;
+// the field spreads them about as well as a multiplicative hash (64-word
+// batches of R-MAT 2^18's B rows: 4.45 lanes on the fullest of 32 banks
+// against 4.35, and 6.07 unhashed) with a bit-field extract and an XOR
+// instead of a quarter-rate 32-bit multiply. A bijection on every 64-word
+// block, so the bitmap needs its word count rounded up to 64.
 static_assert(kBigPassWords % 64 == 0, "a full bitmap pass is whole 64-word blocks");
 
 __device__ __forceinline__ int lds_word(int w)
 {
-    const uint32_t u = (uint32_t)w;
-    return w ^ (int)(__builtin_amdgcn_ubfe(u, 6u, 6u) ^ __builtin_amdgcn_ubfe(u, 11u, 6u));
+    return w ^ (int)__builtin_amdgcn_ubfe((uint32_t)w, 6u, 6u);
 }
 
 template <int NT>
