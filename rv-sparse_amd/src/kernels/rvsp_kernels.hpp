@@ -22,6 +22,8 @@ struct AnalysisCounters {
     long long nnz_total;           // scan total
     int n_long;                    // row statistics: rows left to the long-row pass
     long long seg_total;           // big-row segments (scan of the per-row counts)
+    unsigned long long sym_ops[16];  // products per symbolic bin (stream balance)
+    unsigned long long num_ops[64];  // products per numeric bin (stream balance)
 };
 
 // One block's totals of the row-statistics passes (summed into
@@ -29,6 +31,7 @@ struct AnalysisCounters {
 struct StatPart {
     int hist[16];
     unsigned long long prod, cap;
+    unsigned long long ops[16];  // products per bin
 };
 constexpr int kStatsParts = 4096;  // partials a launch_row_stats call may write
 
@@ -136,13 +139,23 @@ public:
     StreamFan &operator=(const StreamFan &) = delete;
     // stream for the i-th independent launch (the side streams in turn)
     hipStream_t pick(int i) const { return n_ <= 1 ? main_ : side_[i % (n_ - 1)]; }
+    // stream for a launch of `work` units: the one with the least work
+    // queued so far (the caller's stream included), which then holds it
+    hipStream_t take(unsigned long long work);
+    // work already queued on the caller's stream
+    void preload(unsigned long long work) { load_[0] += work; }
 
 private:
     hipStream_t main_;
     int n_ = 1, dev_ = -1;
     hipStream_t side_[3] = {};
     hipEvent_t join_[3] = {};
+    unsigned long long load_[4] = {};
 };
+
+// Bins [lo, hi) with rows, largest `work` first (launch order for
+// StreamFan::take; with no work figures, bin order). Returns how many.
+int bins_by_work(int lo, int hi, const int32_t *count, const unsigned long long *work, int *order);
 // Passes with fewer products run on one stream: the fork and join (about
 // 50 us) would cost more than the launches' overlap saves.
 constexpr int64_t kFanMinProducts = int64_t(1) << 21;
@@ -163,11 +176,14 @@ void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int
                         int32_t *cursor, int32_t *perm, int nb);
 // learned: hash bins use the overflow-checking count (a row whose count
 // exceeds H/2 reports H + 1 instead of raising an error).
+// work: products per symbolic bin (sym_ops), as for launch_numeric.
 void launch_sym_count(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
-                      const int32_t *off, bool learned, bool fan = false);
+                      const int32_t *off, bool learned, bool fan = false,
+                      const unsigned long long *work = nullptr);
 // skip_big: the big rows' columns are left to a numeric pass with cols_out set.
 void launch_sym_fill(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
-                     const int32_t *off, bool skip_big = false, bool fan = false);
+                     const int32_t *off, bool skip_big = false, bool fan = false,
+                     const unsigned long long *work = nullptr);
 void launch_scan(hipStream_t st, int32_t m, const int32_t *cnt, int64_t *tsum, int64_t *total,
                  int32_t *crp);
 int scan_tiles(int32_t m);
@@ -178,8 +194,11 @@ void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_
                      const int32_t *arp, const uint8_t *sbin, const int32_t *lo,
                      const int32_t *hi, const float *chooser, float lg_margin,
                      int64_t products, uint8_t *nbin, AnalysisCounters *ctr);
+// work: products per numeric bin (the analysis counters' num_ops), which
+// spread the bins over the streams largest first; null: bins in turn.
 void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
-                    const int32_t *off, const BigArgs &big, int nsegs, bool fan = false);
+                    const int32_t *off, const BigArgs &big, int nsegs, bool fan = false,
+                    const unsigned long long *work = nullptr);
 // Writes B's (column, value) pairs for a numeric pass: bcv[i] = (bci[i], bval[i]).
 void launch_interleave(hipStream_t st, int32_t nnz, const int32_t *bci, const float *bval,
                        int2 *bcv);
@@ -237,7 +256,7 @@ int pm_scan_tiles(int32_t m);
 void launch_numeric_ranked(hipStream_t st, const NumArgs &a, const int32_t *perm,
                            const int32_t *count, const int32_t *off, const BigArgs &big,
                            int nsegs, const int64_t *prow, const uint8_t *pmap,
-                           bool fan = false);
+                           bool fan = false, const unsigned long long *work = nullptr);
 
 // learned selectors (rvsp_selector.hip)
 const float *selector_device_params(int which, float *margin);

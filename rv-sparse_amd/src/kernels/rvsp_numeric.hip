@@ -489,6 +489,31 @@ StreamFan::StreamFan(hipStream_t st, bool want) : main_(st)
     dev_ = dev;
 }
 
+hipStream_t StreamFan::take(unsigned long long work)
+{
+    int best = 0;
+    for (int i = 1; i < n_; ++i)
+        if (load_[i] < load_[best]) best = i;
+    load_[best] += work;
+    return best == 0 ? main_ : side_[best - 1];
+}
+
+int bins_by_work(int lo, int hi, const int32_t *count, const unsigned long long *work, int *order)
+{
+    int n = 0;
+    for (int b = lo; b < hi; ++b) {
+        if (count[b] <= 0) continue;
+        int j = n++;
+        // insertion by descending work (stable: equal work keeps bin order)
+        while (work && j > 0 && work[order[j - 1]] < work[b]) {
+            order[j] = order[j - 1];
+            --j;
+        }
+        order[j] = b;
+    }
+    return n;
+}
+
 StreamFan::~StreamFan()
 {
     if (dev_ < 0) return;
@@ -502,13 +527,18 @@ StreamFan::~StreamFan()
 }
 
 void launch_numeric(hipStream_t st, const NumArgs &a, const int32_t *perm, const int32_t *count,
-                    const int32_t *off, const BigArgs &big, int nsegs, bool fan)
+                    const int32_t *off, const BigArgs &big, int nsegs, bool fan,
+                    const unsigned long long *work)
 {
-    const StreamFan f(st, fan);
+    StreamFan f(st, fan);
     launch_num_big(st, nsegs, a, big);  // the longest work first, on the caller's stream
-    int i = 0;
-    for (int b = NB_HASH0; b < NB_WIN; ++b)
-        if (count[b] > 0) (void)launch_num_bin(f.pick(i++), a, b, perm + off[b], count[b]);
+    if (work) f.preload(work[NB_WIN]);
+    int order[NB_COUNT];
+    const int n = bins_by_work(NB_HASH0, NB_WIN, count, work, order);
+    for (int i = 0; i < n; ++i) {
+        const int b = order[i];
+        (void)launch_num_bin(work ? f.take(work[b]) : f.pick(i), a, b, perm + off[b], count[b]);
+    }
 }
 
 } // namespace rvsp

@@ -677,8 +677,11 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     // rows without products get no kernel, so their count must start at 0
     HIP_TRY(hipMemsetAsync(p->cnt, 0, M * 4, st));
     const bool sized = sizer && p->hctr.moved > 0;
+    // (the bins' products spread them over the streams; a learned sizing
+    // moves rows after they were counted, so its bins go in turn)
     launch_sym_count(st, sa, p->sperm, p->sym_count, p->sym_off, sized,
-                     fan_out(p, p->products, p->tp.fan_min_products));
+                     fan_out(p, p->products, p->tp.fan_min_products),
+                     sized ? nullptr : p->hctr.sym_ops);
     HIP_TRY(hipGetLastError());
     if (sized) {
         // rows whose learned table overflowed go back to their heuristic
@@ -776,6 +779,13 @@ rvsp_status_t rvsp_hip_plan_analyse(rvsp_hip_plan_t *p, int32_t m, int32_t k, in
     return RVSP_SUCCESS;
 }
 
+// Products per symbolic bin for the fill's stream balance (null after a
+// learned sizing, which moves rows between bins after they were counted).
+static const unsigned long long *bin_ops(const rvsp_hip_plan *p)
+{
+    return p->sized_rows > 0 ? nullptr : p->hctr.sym_ops;
+}
+
 rvsp_status_t rvsp_hip_plan_fill(rvsp_hip_plan_t *p, int32_t *cci)
 {
     if (!p || !cci) return RVSP_ERROR_NULL_POINTER;
@@ -784,7 +794,7 @@ rvsp_status_t rvsp_hip_plan_fill(rvsp_hip_plan_t *p, int32_t *cci)
                p->crp, cci,    p->big, p->err, p->ops, p->bms_spill_ok ? p->bms_spill : nullptr};
     PhaseScope timer(p->stream, PH_FILL);
     launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off, false,
-                    fan_out(p, p->nnzc, p->tp.fan_min_fill));
+                    fan_out(p, p->nnzc, p->tp.fan_min_fill), bin_ops(p));
     HIP_TRY(hipGetLastError());
     return RVSP_SUCCESS;
 }
@@ -808,10 +818,11 @@ static void queue_numeric(rvsp_hip_plan *p, const float *aval, const float *bval
     const NumArgs a = num_args(p, aval, bval, cci, cval);
     if (p->pm_ready)
         launch_numeric_ranked(p->stream, a, p->nperm, p->num_count, p->num_off, big, p->nseg,
-                              p->prow, p->pmap, fan_out(p, p->products, p->tp.fan_min_products));
+                              p->prow, p->pmap, fan_out(p, p->products, p->tp.fan_min_products),
+                              p->hctr.num_ops);
     else
         launch_numeric(p->stream, a, p->nperm, p->num_count, p->num_off, big, p->nseg,
-                       fan_out(p, p->products, p->tp.fan_min_products));
+                       fan_out(p, p->products, p->tp.fan_min_products), p->hctr.num_ops);
 }
 
 rvsp_status_t rvsp_hip_plan_numeric(rvsp_hip_plan_t *p, const float *aval, const float *bval,
@@ -835,7 +846,7 @@ rvsp_status_t rvsp_hip_plan_fill_numeric(rvsp_hip_plan_t *p, const float *aval,
                    p->crp, cci,    p->big, p->err, p->ops, p->bms_spill_ok ? p->bms_spill : nullptr};
         PhaseScope timer(p->stream, PH_FILL);
         launch_sym_fill(p->stream, sa, p->sperm, p->sym_count, p->sym_off, /*skip_big=*/true,
-                        fan_out(p, p->nnzc, p->tp.fan_min_fill));
+                        fan_out(p, p->nnzc, p->tp.fan_min_fill), bin_ops(p));
         HIP_TRY(hipGetLastError());
     }
     PhaseScope timer(p->stream, PH_NUMERIC);

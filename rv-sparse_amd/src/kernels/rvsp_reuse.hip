@@ -368,16 +368,21 @@ int pm_scan_tiles(int32_t m) { return (int)(((int64_t)m + kPmTile - 1) / kPmTile
 
 void launch_numeric_ranked(hipStream_t st, const NumArgs &a, const int32_t *perm,
                            const int32_t *count, const int32_t *off, const BigArgs &big,
-                           int nsegs, const int64_t *prow, const uint8_t *pmap, bool fan)
+                           int nsegs, const int64_t *prow, const uint8_t *pmap, bool fan,
+                           const unsigned long long *work)
 {
     // bins touch disjoint rows: big rows first on the caller's stream, the
-    // others over the side streams (none under graph capture)
-    const StreamFan f(st, fan);
+    // others over the streams largest first (none under graph capture)
+    StreamFan f(st, fan);
     launch_num_big(st, nsegs, a, big);
-    int i = 0;
-    for (int b = NB_HASH0; b < NB_WIN; ++b)
-        if (count[b] > 0)
-            (void)launch_ranked_bin(f.pick(i++), a, b, perm + off[b], count[b], prow, pmap);
+    if (work) f.preload(work[NB_WIN]);
+    int order[NB_COUNT];
+    const int n = bins_by_work(NB_HASH0, NB_WIN, count, work, order);
+    for (int i = 0; i < n; ++i) {
+        const int b = order[i];
+        (void)launch_ranked_bin(work ? f.take(work[b]) : f.pick(i), a, b, perm + off[b], count[b],
+                                prow, pmap);
+    }
 }
 
 } // namespace rvsp

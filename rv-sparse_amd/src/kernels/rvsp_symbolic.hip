@@ -126,8 +126,11 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, int32_t nnz_a, int
 {
     constexpr int G = 4, U = 8;
     __shared__ int hist[SB_COUNT];
-    __shared__ unsigned long long bprod, bcap;
-    if (threadIdx.x < SB_COUNT) hist[threadIdx.x] = 0;
+    __shared__ unsigned long long bprod, bcap, bops[SB_COUNT];
+    if (threadIdx.x < SB_COUNT) {
+        hist[threadIdx.x] = 0;
+        bops[threadIdx.x] = 0ull;
+    }
     if (threadIdx.x == 0) bprod = bcap = 0ull;
     __syncthreads();
     const int lane = threadIdx.x & 63, gl = lane % G;
@@ -188,12 +191,18 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, int32_t nnz_a, int
             sbin_out[row] = (uint8_t)b;
             atomicAdd(&hist[b], 1);
             if (b == SB_BIG) atomicAdd(&bcap, (unsigned long long)big_nchunks(lo, hi) * kChunkWords);
-            if (ops) atomicAdd(&bprod, (unsigned long long)ops);
+            if (ops) {
+                atomicAdd(&bprod, (unsigned long long)ops);
+                atomicAdd(&bops[b], (unsigned long long)ops);
+            }
         }
     }
     __syncthreads();
     StatPart &pp = parts[blockIdx.x];
-    if (threadIdx.x < SB_COUNT) pp.hist[threadIdx.x] = hist[threadIdx.x];
+    if (threadIdx.x < SB_COUNT) {
+        pp.hist[threadIdx.x] = hist[threadIdx.x];
+        pp.ops[threadIdx.x] = bops[threadIdx.x];
+    }
     if (threadIdx.x == 0) {
         pp.prod = bprod;
         pp.cap = bcap;
@@ -213,7 +222,11 @@ __global__ __launch_bounds__(kStatsLongThreads) void k_row_stats_long(
     __shared__ long long s_ops[NWV];
     __shared__ int s_lo[NWV], s_hi[NWV];
     __shared__ int hist[SB_COUNT];
-    if (threadIdx.x < SB_COUNT) hist[threadIdx.x] = 0;
+    __shared__ unsigned long long bops[SB_COUNT];
+    if (threadIdx.x < SB_COUNT) {
+        hist[threadIdx.x] = 0;
+        bops[threadIdx.x] = 0ull;
+    }
     __syncthreads();
     unsigned long long bprod = 0, bcap = 0;  // thread 0's
     const int n = ctr->n_long;
@@ -247,13 +260,17 @@ __global__ __launch_bounds__(kStatsLongThreads) void k_row_stats_long(
             row_stats_finish(row, ops, lo, hi, ops_out, lo_out, hi_out);
             sbin_out[row] = (uint8_t)b;
             ++hist[b];
+            bops[b] += (unsigned long long)ops;
             if (b == SB_BIG) bcap += (unsigned long long)big_nchunks(lo, hi) * kChunkWords;
             bprod += (unsigned long long)ops;
         }
         __syncthreads();  // the block's partials are read before the next row's
     }
     StatPart &pp = parts[blockIdx.x];
-    if (threadIdx.x < SB_COUNT) pp.hist[threadIdx.x] = hist[threadIdx.x];
+    if (threadIdx.x < SB_COUNT) {
+        pp.hist[threadIdx.x] = hist[threadIdx.x];
+        pp.ops[threadIdx.x] = bops[threadIdx.x];
+    }
     if (threadIdx.x == 0) {
         pp.prod = bprod;
         pp.cap = bcap;
@@ -265,14 +282,17 @@ __global__ __launch_bounds__(kStatsLongThreads) void k_row_stats_long(
 __global__ __launch_bounds__(256) void k_stats_reduce(const StatPart *__restrict__ parts, int n,
                                                       AnalysisCounters *__restrict__ ctr)
 {
-    constexpr int NV = SB_COUNT + 2;
+    constexpr int NV = 2 * SB_COUNT + 2;  // histogram, products, words, products per bin
     __shared__ unsigned long long s_v[4][NV];
     unsigned long long v[NV];
 #pragma unroll
     for (int b = 0; b < NV; ++b) v[b] = 0ull;
     for (int i = threadIdx.x; i < n; i += 256) {
 #pragma unroll
-        for (int b = 0; b < SB_COUNT; ++b) v[b] += (unsigned long long)parts[i].hist[b];
+        for (int b = 0; b < SB_COUNT; ++b) {
+            v[b] += (unsigned long long)parts[i].hist[b];
+            v[SB_COUNT + 2 + b] += parts[i].ops[b];
+        }
         v[SB_COUNT] += parts[i].prod;
         v[SB_COUNT + 1] += parts[i].cap;
     }
@@ -292,8 +312,10 @@ __global__ __launch_bounds__(256) void k_stats_reduce(const StatPart *__restrict
             ctr->sym_hist[threadIdx.x] = (int)t;
         else if (threadIdx.x == SB_COUNT)
             ctr->products = t;
-        else
+        else if (threadIdx.x == SB_COUNT + 1)
             ctr->bits_cap = t;
+        else
+            ctr->sym_ops[threadIdx.x - SB_COUNT - 2] = t;
     }
 }
 
@@ -770,13 +792,18 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
                                                   AnalysisCounters *__restrict__ ctr)
 {
     __shared__ int hist[NB_COUNT];
+    __shared__ unsigned long long bops[NB_COUNT];
     __shared__ float P[Chooser::NP];
-    if (threadIdx.x < NB_COUNT) hist[threadIdx.x] = 0;
+    if (threadIdx.x < NB_COUNT) {
+        hist[threadIdx.x] = 0;
+        bops[threadIdx.x] = 0ull;
+    }
     if (chooser)
         for (int p = threadIdx.x; p < Chooser::NP; p += 256) P[p] = chooser[p];
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < m) {
+    // blocks stride over the rows, so the counters take one global atomic
+    // per bin per block of a bounded grid
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256) {
         const int nz = crp[i + 1] - crp[i];
         int b;
         if (nz == 0)
@@ -796,10 +823,13 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
         }
         nbin[i] = (uint8_t)b;
         atomicAdd(&hist[b], 1);
+        if (ops[i]) atomicAdd(&bops[b], (unsigned long long)ops[i]);
     }
     __syncthreads();
-    if (threadIdx.x < NB_COUNT && hist[threadIdx.x])
+    if (threadIdx.x < NB_COUNT && hist[threadIdx.x]) {
         atomicAdd(&ctr->num_hist[threadIdx.x], hist[threadIdx.x]);
+        if (bops[threadIdx.x]) atomicAdd(&ctr->num_ops[threadIdx.x], bops[threadIdx.x]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -839,18 +869,25 @@ static void launch_sym_learned(hipStream_t st, const SymArgs &a, int b, const in
 template <bool FILL>
 static void launch_sym_bins(hipStream_t main, const SymArgs &a, const int32_t *perm,
                             const int32_t *count, const int32_t *off, bool learned = false,
-                            bool skip_big = false, bool fan = false)
+                            bool skip_big = false, bool fan = false,
+                            const unsigned long long *work = nullptr)
 {
     // bins touch disjoint rows: the big rows (the longest work) first on the
-    // caller's stream, the others over the side streams
-    const StreamFan f(main, fan);
+    // caller's stream, the others over the streams, largest first onto the
+    // least loaded when the bins' products are known, else in turn over the
+    // side streams
+    StreamFan f(main, fan);
     static_assert(SB_BIG == SB_COUNT - 1 && SB_BMS == SB_BIG - 1, "big rows are the last bin");
-    int i = 0;
-    for (int bi = 0; bi < SB_COUNT - SB_H64; ++bi) {
-        const int b = bi == 0 ? SB_BIG : SB_H64 + bi - 1;  // SB_BIG, then SB_H64..SB_BMS
+    const bool big = count[SB_BIG] > 0 && !skip_big;
+    if (work && big) f.preload(work[SB_BIG]);
+    int order[SB_COUNT];
+    order[0] = SB_BIG;
+    const int n = (big ? 1 : 0) + bins_by_work(SB_H64, SB_BIG, count, work, order + (big ? 1 : 0));
+    for (int bi = 0; bi < n; ++bi) {
+        const int b = order[bi];
         const int nr = count[b];
-        if (nr == 0 || (skip_big && b == SB_BIG)) continue;
-        const hipStream_t st = b == SB_BIG ? main : f.pick(i++);
+        const hipStream_t st =
+            b == SB_BIG ? main : work ? f.take(work[b]) : f.pick(bi - (big ? 1 : 0));
         const int32_t *rows = perm + off[b];
         if (!FILL && learned && b >= SB_H64 && b <= SB_H4096) {
             launch_sym_learned(st, a, b, rows, nr);
@@ -912,15 +949,15 @@ void launch_bin_scatter(hipStream_t st, int32_t m, const uint8_t *bin, const int
 }
 
 void launch_sym_count(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
-                      const int32_t *off, bool learned, bool fan)
+                      const int32_t *off, bool learned, bool fan, const unsigned long long *work)
 {
-    launch_sym_bins<false>(st, a, perm, count, off, learned, false, fan);
+    launch_sym_bins<false>(st, a, perm, count, off, learned, false, fan, work);
 }
 
 void launch_sym_fill(hipStream_t st, const SymArgs &a, const int32_t *perm, const int32_t *count,
-                     const int32_t *off, bool skip_big, bool fan)
+                     const int32_t *off, bool skip_big, bool fan, const unsigned long long *work)
 {
-    launch_sym_bins<true>(st, a, perm, count, off, false, skip_big, fan);
+    launch_sym_bins<true>(st, a, perm, count, off, false, skip_big, fan, work);
 }
 
 void launch_scan(hipStream_t st, int32_t m, const int32_t *cnt, int64_t *tsum, int64_t *total,
@@ -944,7 +981,8 @@ void launch_num_bins(hipStream_t st, int32_t m, const int32_t *crp, const int64_
                      const int32_t *hi, const float *chooser, float lg_margin,
                      int64_t products, uint8_t *nbin, AnalysisCounters *ctr)
 {
-    const int grid = (int)(((int64_t)m + 255) / 256);
+    if (m <= 0) return;
+    const int grid = (int)std::min<int64_t>(((int64_t)m + 255) / 256, 512);
     k_num_bins<<<grid, 256, 0, st>>>(m, crp, ops, arp, sbin, lo, hi, chooser, lg_margin, products,
                                      nbin, ctr);
 }
