@@ -142,8 +142,13 @@ public:
     // stream for a launch of `work` units: the one with the least work
     // queued so far (the caller's stream included), which then holds it
     hipStream_t take(unsigned long long work);
-    // work already queued on the caller's stream
+    // big-row work already queued on the caller's stream
     void preload(unsigned long long work) { load_[0] += work; }
+    // streams for the bins [lo, hi) with rows, in launch order: with work
+    // figures and no big rows queued, largest first by take(); else bin
+    // order over the side streams in turn. Returns how many.
+    int assign(int lo, int hi, const int32_t *count, const unsigned long long *work, int *order,
+               hipStream_t *sts);
 
 private:
     hipStream_t main_;

@@ -377,12 +377,11 @@ void launch_numeric_ranked(hipStream_t st, const NumArgs &a, const int32_t *perm
     launch_num_big(st, nsegs, a, big);
     if (work) f.preload(work[NB_WIN]);
     int order[NB_COUNT];
-    const int n = bins_by_work(NB_HASH0, NB_WIN, count, work, order);
-    for (int i = 0; i < n; ++i) {
-        const int b = order[i];
-        (void)launch_ranked_bin(work ? f.take(work[b]) : f.pick(i), a, b, perm + off[b], count[b],
-                                prow, pmap);
-    }
+    hipStream_t sts[NB_COUNT];
+    const int n = f.assign(NB_HASH0, NB_WIN, count, work, order, sts);
+    for (int i = 0; i < n; ++i)
+        (void)launch_ranked_bin(sts[i], a, order[i], perm + off[order[i]], count[order[i]], prow,
+                                pmap);
 }
 
 } // namespace rvsp

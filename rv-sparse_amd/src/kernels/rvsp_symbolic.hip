@@ -881,13 +881,15 @@ static void launch_sym_bins(hipStream_t main, const SymArgs &a, const int32_t *p
     const bool big = count[SB_BIG] > 0 && !skip_big;
     if (work && big) f.preload(work[SB_BIG]);
     int order[SB_COUNT];
+    hipStream_t sts[SB_COUNT];
     order[0] = SB_BIG;
-    const int n = (big ? 1 : 0) + bins_by_work(SB_H64, SB_BIG, count, work, order + (big ? 1 : 0));
+    sts[0] = main;
+    const int n = (big ? 1 : 0) + f.assign(SB_H64, SB_BIG, count, work, order + (big ? 1 : 0),
+                                           sts + (big ? 1 : 0));
     for (int bi = 0; bi < n; ++bi) {
         const int b = order[bi];
         const int nr = count[b];
-        const hipStream_t st =
-            b == SB_BIG ? main : work ? f.take(work[b]) : f.pick(bi - (big ? 1 : 0));
+        const hipStream_t st = sts[bi];
         const int32_t *rows = perm + off[b];
         if (!FILL && learned && b >= SB_H64 && b <= SB_H4096) {
             launch_sym_learned(st, a, b, rows, nr);
