@@ -793,14 +793,14 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
 {
     __shared__ int hist[NB_COUNT];
     __shared__ unsigned long long bops[NB_COUNT];
-    __shared__ float P[Chooser::NP];
     if (threadIdx.x < NB_COUNT) {
         hist[threadIdx.x] = 0;
         bops[threadIdx.x] = 0ull;
     }
-    if (chooser)
-        for (int p = threadIdx.x; p < Chooser::NP; p += 256) P[p] = chooser[p];
     __syncthreads();
+    // (the chooser's weights are read from its global parameters at
+    // wave-uniform addresses, so they come through scalar loads into SGPRs
+    // instead of one LDS read per multiply-add)
     // blocks stride over the rows, so the counters take one global atomic
     // per bin per block of a bounded grid
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256) {
@@ -818,7 +818,7 @@ __global__ __launch_bounds__(256) void k_num_bins(int32_t m, const int32_t *__re
             // the chooser (trained on throughput per row) is not asked
             const bool critical = ops[i] * kCriticalShare >= products;
             const int hb = num_bin_of(nz, ops[i], alen, span, critical);
-            b = chooser && !critical ? chooser_pick(P, ops[i], alen, span, nz, hb, lg_margin) : -1;
+            b = chooser && !critical ? chooser_pick(chooser, ops[i], alen, span, nz, hb, lg_margin) : -1;
             if (b < 0) b = hb;
         }
         nbin[i] = (uint8_t)b;
