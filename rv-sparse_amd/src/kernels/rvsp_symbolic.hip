@@ -20,9 +20,19 @@ namespace rvsp {
 // 16 lanes per row; lanes stride over the row's A entries.
 // ---------------------------------------------------------------------------
 // hmax (tuned, at most kSymHashMaxU): rows with u <= hmax use a hash table
+// Rows of at least kBigRowOps products take the big-row path whatever their
+// span: its count spreads a row over a 512-thread block and its numeric
+// pass over one wave per window run, where the one-pass bitmap and dense
+// kernels give a row one 64-lane group. (Generated 16384^2: every row's
+// span fits the one-pass bitmap, and its near-dense rows held the count at
+// 3.6 ms and the numeric pass at 7.3 ms: 48 -> 157 GFLOP/s at 8192, 131 at
+// 2^17; 2048 moves the 512-product batch's rows too, 25.7 -> 28.1 ms.)
+constexpr int64_t kBigRowOps = 8192;
+
 __device__ __forceinline__ int sym_bin_of(int64_t ops, int32_t lo, int32_t hi, int hmax)
 {
     if (ops == 0) return SB_EMPTY;
+    if (ops >= kBigRowOps) return SB_BIG;
     const int64_t span = (int64_t)hi - lo + 1;
     const int64_t u = ops < span ? ops : span;
     int H = 64;
