@@ -21,18 +21,25 @@ namespace rvsp {
 // ---------------------------------------------------------------------------
 // hmax (tuned, at most kSymHashMaxU): rows with u <= hmax use a hash table
 // Rows of at least kBigRowOps products take the big-row path whatever their
-// span: its count spreads a row over a 512-thread block and its numeric
-// pass over one wave per window run, where the one-pass bitmap and dense
-// kernels give a row one 64-lane group. (Generated 16384^2: every row's
-// span fits the one-pass bitmap, and its near-dense rows held the count at
-// 3.6 ms and the numeric pass at 7.3 ms: 48 -> 157 GFLOP/s at 8192, 131 at
-// 2^17; 2048 moves the 512-product batch's rows too, 25.7 -> 28.1 ms.)
+// span, once A has kBigRowNnz entries: its count spreads a row over a
+// 512-thread block and its numeric pass over one wave per window run, where
+// the one-pass bitmap and dense kernels give a row one 64-lane group.
+// (Generated 16384^2: every row's span fits the one-pass bitmap, and its
+// near-dense rows held the count at 3.6 ms and the numeric pass at 7.3 ms:
+// 48 -> 157 GFLOP/s at 8192, 131 at 2^17; 2048 moves the 512-product
+// batch's rows too, 25.7 -> 28.1 ms. Generated 2048^2 to 8192^2, with 94 k
+// entries and more: 1.16x to 1.38x; 512^2 and 1024^2, with 15 k and 41 k
+// entries: 0.72x and 0.87x, the big-row path's launches outweighing what it
+// spreads.)
 constexpr int64_t kBigRowOps = 8192;
+constexpr int32_t kBigRowNnz = 65536;
 
-__device__ __forceinline__ int sym_bin_of(int64_t ops, int32_t lo, int32_t hi, int hmax)
+// big_ops: kBigRowOps, or INT64_MAX below kBigRowNnz entries of A
+__device__ __forceinline__ int sym_bin_of(int64_t ops, int32_t lo, int32_t hi, int hmax,
+                                          int64_t big_ops)
 {
     if (ops == 0) return SB_EMPTY;
-    if (ops >= kBigRowOps) return SB_BIG;
+    if (ops >= big_ops) return SB_BIG;
     const int64_t span = (int64_t)hi - lo + 1;
     const int64_t u = ops < span ? ops : span;
     int H = 64;
@@ -130,7 +137,7 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, int32_t nnz_a, int
                                                    int32_t *__restrict__ hi_out,
                                                    uint8_t *__restrict__ sbin_out,
                                                    AnalysisCounters *__restrict__ ctr,
-                                                   int hash_max_u,
+                                                   int hash_max_u, int64_t big_ops,
                                                    int32_t *__restrict__ long_rows,
                                                    StatPart *__restrict__ parts)
 {
@@ -196,7 +203,7 @@ __global__ __launch_bounds__(256) void k_row_stats(int32_t m, int32_t nnz_a, int
             hi = max(hi, __shfl_xor(hi, off, G));
         }
         if (row < m && gl == 0 && !lng) {
-            const int b = sym_bin_of(ops, lo, hi, hash_max_u);
+            const int b = sym_bin_of(ops, lo, hi, hash_max_u, big_ops);
             row_stats_finish(row, ops, lo, hi, ops_out, lo_out, hi_out);
             sbin_out[row] = (uint8_t)b;
             atomicAdd(&hist[b], 1);
@@ -226,7 +233,8 @@ __global__ __launch_bounds__(kStatsLongThreads) void k_row_stats_long(
     int32_t nnz_a, int32_t kb, const int32_t *__restrict__ arp, const int32_t *__restrict__ aci, const int4 *__restrict__ bst,
     const int32_t *__restrict__ long_rows, int64_t *__restrict__ ops_out,
     int32_t *__restrict__ lo_out, int32_t *__restrict__ hi_out, uint8_t *__restrict__ sbin_out,
-    const AnalysisCounters *__restrict__ ctr, int hash_max_u, StatPart *__restrict__ parts)
+    const AnalysisCounters *__restrict__ ctr, int hash_max_u, int64_t big_ops,
+    StatPart *__restrict__ parts)
 {
     constexpr int U = 8, NWV = kStatsLongThreads / 64;
     __shared__ long long s_ops[NWV];
@@ -266,7 +274,7 @@ __global__ __launch_bounds__(kStatsLongThreads) void k_row_stats_long(
                 lo = min(lo, s_lo[w]);
                 hi = max(hi, s_hi[w]);
             }
-            const int b = sym_bin_of(ops, lo, hi, hash_max_u);
+            const int b = sym_bin_of(ops, lo, hi, hash_max_u, big_ops);
             row_stats_finish(row, ops, lo, hi, ops_out, lo_out, hi_out);
             sbin_out[row] = (uint8_t)b;
             ++hist[b];
@@ -941,13 +949,15 @@ void launch_row_stats(hipStream_t st, int32_t m, int32_t nnz_a, const int32_t *a
     const int64_t ngroups = ((int64_t)m + kStatsRowsPerBlock - 1) / kStatsRowsPerBlock;
     const int g1 = (int)std::max<int64_t>(1, std::min<int64_t>(ngroups, kStatsBlocks));
     const int hmax = hash_max_u < 64 ? 64 : hash_max_u > kSymHashMaxU ? kSymHashMaxU : hash_max_u;
+    const int64_t big_ops = nnz_a >= kBigRowNnz ? kBigRowOps : INT64_MAX;
     k_row_stats<<<g1, 256, 0, st>>>(m, nnz_a, k, arp, aci, bst, ops, lo, hi, sbin, ctr, hmax,
-                                    long_rows, parts);
+                                    big_ops, long_rows, parts);
     int g2 = 0;
     if (nnz_a > kStatsLong) {  // else no row can be long
         g2 = std::min(nnz_a / (kStatsLong + 1), kStatsBlocks);  // listed rows at most
         k_row_stats_long<<<g2, kStatsLongThreads, 0, st>>>(nnz_a, k, arp, aci, bst, long_rows, ops,
-                                                           lo, hi, sbin, ctr, hmax, parts + g1);
+                                                           lo, hi, sbin, ctr, hmax, big_ops,
+                                                           parts + g1);
     }
     k_stats_reduce<<<1, 256, 0, st>>>(parts, g1 + g2, ctr);
 }
