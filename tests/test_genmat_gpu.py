@@ -43,6 +43,8 @@ def test_generated_pairs_match_oracle(R, Cc, D, S):
 
 # larger generated pairs: log-normal degrees put whole rows of C at full
 # width (16384, 32768 columns), so they cross the dense and big-row paths
+# (the 16384^2 pair's near-dense rows fit the one-pass bitmap's span but
+# take the big-row path by their product count)
 LARGE_CASES = [(16384, 16384, 0.001, 7), (32768, 32768, 0.0005, 9), (8192, 40000, 0.0008, 3)]
 
 
