@@ -35,7 +35,7 @@ CANDIDATES = {
     "hash_max_u": [256, 512, 1024, 2048],
     "big_window": [(2048, 16), (4096, 32), (2048, 32), (1024, 16), (1024, 8)],
     "sym_big_threads": [256, 512, 1024],
-    "seg_products": [8192, 16384, 32768],
+    "seg_products": [8192, 16384, 32768, 65536, 98304],
     "fan_min_products": [1 << 19, 1 << 21, 1 << 23, 1 << 62],
     "fan_min_fill": [1 << 22, 1 << 24, 1 << 26, 1 << 62],
 }
