@@ -31,6 +31,16 @@ namespace rvsp {
 // entries and more: 1.16x to 1.38x; 512^2 and 1024^2, with 15 k and 41 k
 // entries: 0.72x and 0.87x, the big-row path's launches outweighing what it
 // spreads.)
+// Rows of more candidate columns u than the class's hash limit, too wide
+// for the one-pass bitmap, take the big-row path unless their span is at
+// most kBigSpanFactor * u (many repeated products over a narrow band) and
+// u fits the largest hash table: a 27-point stencil 64^3 row bounds 729
+// candidates over about 16.6 k columns (125 outputs) and stays in the hash
+// tables, where R-MAT's and the mixed product's rows of that size span
+// 2^18 to 2^20 columns and move. (With a hash limit of 512 in their class,
+// the stencil held 1300 GFLOP/s with the span test and fell to 333
+// without it; at the shipped limits the test moves no measured input.)
+constexpr int64_t kBigSpanFactor = 64;
 constexpr int64_t kBigRowOps = 8192;
 constexpr int32_t kBigRowNnz = 65536;
 
@@ -46,6 +56,8 @@ __device__ __forceinline__ int sym_bin_of(int64_t ops, int32_t lo, int32_t hi, i
     if (u <= hmax) H = max(64, next_pow2((int)(2 * u)));
     if (span <= 32 * kBmsWords && (u > hmax || span <= 32 * (int64_t)H)) return SB_BMS;
     if (u <= hmax) return SB_H64 + (ilog2(H) - 6);
+    if (u <= kSymHashMaxU && span <= kBigSpanFactor * u)
+        return SB_H64 + (ilog2(max(64, next_pow2((int)(2 * u)))) - 6);
     return SB_BIG;
 }
 
