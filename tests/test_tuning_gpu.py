@@ -79,3 +79,35 @@ def test_table_entry_applies_to_its_class(cases):
         assert_bitexact(got, rp, ci, va, f"class {cls} entry set")
     finally:
         assert rvsp.lib().rvsp_tuning_set(cls, rvsp.Tuning.of(before)) == 0
+
+
+def test_narrow_band_rows_over_the_hash_limit():
+    """Rows of more candidate columns than the hash limit, too wide for the
+    one-pass bitmap (16384 columns) but spanning at most 64 columns a
+    candidate, stay in the hash tables (interior rows of a 27-point stencil
+    64^3 times the whole stencil: 729 candidates over about 16.6 k
+    columns); C is the oracle's, bit for bit, with the limit below and
+    above their candidate count."""
+    S, _ = W.stencil(g=64, seed=98)
+    # interior rows of the z = 32 plane (x, y in 2..61: all 125 neighbours)
+    rows = np.array([32 * 4096 + y * 64 + x for y in range(2, 10) for x in range(2, 62)])
+    parts = [np.arange(S.row_ptr[r], S.row_ptr[r + 1]) for r in rows]
+    at = np.concatenate(parts)
+    rptr = np.zeros(len(rows) + 1, np.int32)
+    rptr[1:] = np.cumsum([len(q) for q in parts])
+    A = rvsp.Csr(len(rows), S.cols, rptr, np.ascontiguousarray(S.col_idx[at]),
+                 np.ascontiguousarray(S.values[at]))
+    # the rows reach the span test: u = min(products, span) > 512,
+    # 16384 < span <= 64 u
+    blen = np.diff(S.row_ptr)
+    bfirst, blast = S.col_idx[S.row_ptr[:-1]], S.col_idx[S.row_ptr[1:] - 1]
+    for r in range(A.rows):
+        ks = A.col_idx[A.row_ptr[r]:A.row_ptr[r + 1]]
+        ops, span = int(blen[ks].sum()), int(blast[ks].max() - bfirst[ks].min() + 1)
+        u = min(ops, span)
+        assert u > 512 and 16384 < span <= 64 * u, (r, ops, span)
+    rp, ci, va = pyoracle.spgemm(A, S)[:3]
+    for hmax in (256, 512, 2048):
+        print(f"stencil band rows, hash limit {hmax}", flush=True)
+        run_all([("stencil band", A, S, (rp, ci, va))], dict(rvsp.tuning_defaults(), hash_max_u=hmax),
+                f"hash limit {hmax}")
